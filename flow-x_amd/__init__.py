@@ -1,0 +1,28 @@
+"""flowx_amd - MI355X-native Position Based Fluids step / reset / readback /
+surface path behind the C ABI of include/flowx.h (libflowx_hip.so, gfx950).
+
+The directory is named flow-x_amd; load it as the package ``flowx_amd`` with
+``__graft_entry__.load_package()`` (or add a spec loader of your own)."""
+
+from . import _lib
+from .collision import ColliderGrid, grid_geometry
+from .config import (
+    DomainSettings,
+    SolverConfig,
+    adjoint_config,
+    animated_config,
+    batch_configs,
+    dam_break_config,
+    lattice_config,
+    resolve_config,
+    scale_config,
+    seed_positions,
+)
+from .sph import Solver
+from .surface import MarchingCubes, Surface, SurfaceConfig, resolve
+
+__all__ = [
+    "ColliderGrid", "DomainSettings", "MarchingCubes", "Solver", "SolverConfig", "Surface", "SurfaceConfig",
+    "adjoint_config", "animated_config", "batch_configs", "dam_break_config", "grid_geometry",
+    "lattice_config", "resolve", "resolve_config", "scale_config", "seed_positions",
+]

@@ -1,0 +1,106 @@
+"""Collider occupancy grid: geometry and GPU voxelization of analytic shapes.
+
+Geometry follows collision/__init__.py:245-260 (_domain_grid_geometry): the
+grid shares the domain's low corner as origin, voxel = longest axis /
+resolution, dims = max(1, round(axis / voxel)).  Occupancy is decided at voxel
+centres origin + (i + 0.5) * voxel and stored uint8 at (k*ny + j)*nx + i, the
+union over every collider (collision/__init__.py:225-242, :353-366).  The
+reference voxelizes Blender meshes by BVH ray parity; this path voxelizes the
+analytic shapes its benchmark scenes use (boxes, a vertical cylinder, a
+sphere) on the GPU, fast enough to redo every frame for a moving collider.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import f32, vec32
+
+
+def grid_geometry(lo, hi, resolution):
+    """(origin, voxel, dims) - collision/__init__.py:245-255."""
+    lo, hi = vec32(lo), vec32(hi)
+    size = tuple(f32(b - a) for a, b in zip(lo, hi))
+    longest = max(size)
+    if longest <= 0.0:
+        return lo, 0.0, (0, 0, 0)
+    voxel = longest / resolution
+    dims = tuple(max(1, round(axis / voxel)) for axis in size)
+    return lo, voxel, dims
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(c) for c in v])
+
+
+def _i3(v):
+    return (ctypes.c_int32 * 3)(*[int(c) for c in v])
+
+
+class ColliderGrid:
+    """Device uint8 occupancy grid for one domain."""
+
+    def __init__(self, lo, hi, resolution, device="cuda"):
+        self.origin, self.voxel, self.dims = grid_geometry(lo, hi, resolution)
+        nx, ny, nz = self.dims
+        self.grid = torch.zeros(nz * ny * nx, dtype=torch.uint8, device=device)
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.grid.device).cuda_stream)
+
+    def clear(self):
+        self.grid.zero_()
+
+    def add(self, shape, accumulate=True):
+        """OR one analytic shape into the grid.  shape: ("box", lo, hi) |
+        ("sphere", centre, r) | ("cylinder", base, r, height)."""
+        kind = shape[0]
+        d, o, v = _i3(self.dims), _f3(self.origin), ctypes.c_float(self.voxel)
+        g = ctypes.c_void_p(self.grid.data_ptr())
+        acc = 1 if accumulate else 0
+        if kind == "box":
+            boxes = (ctypes.c_float * 6)(*[float(c) for c in (*shape[1], *shape[2])])
+            _lib.call("fx_voxelize_boxes", d, o, v, boxes, 1, g, acc, self._stream())
+        elif kind == "sphere":
+            _lib.call("fx_voxelize_sphere", d, o, v, _f3(shape[1]), ctypes.c_float(shape[2]), g, acc,
+                      self._stream())
+        elif kind == "cylinder":
+            _lib.call("fx_voxelize_cylinder", d, o, v, _f3(shape[1]), ctypes.c_float(shape[2]),
+                      ctypes.c_float(shape[3]), g, acc, self._stream())
+        else:
+            raise ValueError(f"unknown collider shape {kind!r}")
+
+    def rebuild(self, shapes):
+        """Clear, then union every shape (the solver grid of
+        collision/__init__.py:225-242)."""
+        self.clear()
+        for shape in shapes:
+            self.add(shape, accumulate=True)
+
+    def occupancy(self):
+        nx, ny, nz = self.dims
+        return self.grid.cpu().numpy().reshape(nz, ny, nx)
+
+
+def occupancy_cpu(origin, voxel, dims, shapes):
+    """numpy reference of the voxel-centre test (used by host-side callers that
+    have no GPU, e.g. cache hashing of shape lists)."""
+    nx, ny, nz = dims
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x = origin[0] + (i + 0.5) * voxel
+    y = origin[1] + (j + 0.5) * voxel
+    z = origin[2] + (k + 0.5) * voxel
+    occ = np.zeros((nz, ny, nx), bool)
+    for shape in shapes:
+        if shape[0] == "box":
+            a, b = shape[1], shape[2]
+            occ |= (x > a[0]) & (x < b[0]) & (y > a[1]) & (y < b[1]) & (z > a[2]) & (z < b[2])
+        elif shape[0] == "sphere":
+            c, r = shape[1], shape[2]
+            occ |= (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 < r * r
+        elif shape[0] == "cylinder":
+            c, r, hgt = shape[1], shape[2], shape[3]
+            occ |= ((x - c[0]) ** 2 + (y - c[1]) ** 2 < r * r) & (z > c[2]) & (z < c[2] + hgt)
+    return occ.astype(np.uint8)

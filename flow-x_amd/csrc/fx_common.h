@@ -1,0 +1,167 @@
+// Shared device helpers and host plumbing for libflowx_hip.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "flowx.h"
+
+namespace fx {
+
+// ---- error handling (thread-local last error, returned as codes) ---------
+void set_error(const std::string& msg);
+int fail(int code, const char* fmt, ...);
+int check_launch(const char* what);
+
+#define FX_HIP_CHECK(expr)                                                     \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return ::fx::fail(FX_EHIP, "%s: %s", #expr, hipGetErrorString(e_));      \
+  } while (0)
+
+#define FX_TRY(expr)              \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_ != FX_OK) return rc_; \
+  } while (0)
+
+constexpr float kPi = 3.14159265358979323846f;  // GLSL float literal PI
+
+// Derived per-launch constants, computed on the host in float exactly as the
+// GLSL computes them per invocation (shaders/sph_common.glsl:68-81).
+struct Consts {
+  float h, h2, mass, rho0, eps;
+  float poly6, spiky, visc_lap;
+  float w_q;          // W_poly6((0.2h)^2) of shaders/sph_delta.glsl:44
+  float scorr_k;
+  float visc;
+  float dt;
+  int max_scan;
+};
+
+Consts make_consts(const FxParams& p);
+float poly6_coef(float h);
+
+// ---- grid geometry --------------------------------------------------------
+struct Grid {
+  float lox, loy, loz, cell;
+  int cx, cy, cz, nc;
+};
+
+inline Grid make_grid(const FxParams& p) {
+  return Grid{p.lo[0], p.lo[1], p.lo[2], p.cell_size,
+              p.cells[0], p.cells[1], p.cells[2], p.n_cells};
+}
+
+// shaders/sph_common.glsl:50-54 cell_coord: clamp(ivec3(floor((p-lo)/cell)))
+// with IEEE division (no contraction possible: sub, div, floor).
+__device__ __forceinline__ int floor_to_int(float v) {
+  float f = floorf(v);
+  // keep the conversion defined for huge / NaN values; clamp() follows anyway
+  f = fminf(fmaxf(f, -1.0e9f), 1.0e9f);
+  return (int)f;
+}
+
+__device__ __forceinline__ void cell_coord(const Grid& g, float px, float py, float pz,
+                                           int& x, int& y, int& z) {
+  x = min(max(floor_to_int((px - g.lox) / g.cell), 0), g.cx - 1);
+  y = min(max(floor_to_int((py - g.loy) / g.cell), 0), g.cy - 1);
+  z = min(max(floor_to_int((pz - g.loz) / g.cell), 0), g.cz - 1);
+}
+
+__device__ __forceinline__ int cell_index(const Grid& g, int x, int y, int z) {
+  return (z * g.cy + y) * g.cx + x;
+}
+
+// ---- collider (shaders/sph_common.glsl:132-160) ---------------------------
+struct Collider {
+  const uint8_t* occ;  // NULL = none
+  float lox, loy, loz, voxel;
+  int kx, ky, kz;
+};
+
+inline Collider make_collider(const FxParams& p, const uint8_t* occ) {
+  Collider c{occ, p.lo[0], p.lo[1], p.lo[2], p.collider_voxel,
+             p.collider_dims[0], p.collider_dims[1], p.collider_dims[2]};
+  if (!(p.collider_voxel > 0.0f)) c.occ = nullptr;
+  return c;
+}
+
+__device__ __forceinline__ bool collider_occupied(const Collider& c, int x, int y, int z) {
+  if (x < 0 || y < 0 || z < 0 || x >= c.kx || y >= c.ky || z >= c.kz) return false;
+  return c.occ[((size_t)z * c.ky + y) * c.kx + x] != 0;
+}
+
+__device__ __forceinline__ void collider_coord(const Collider& c, float px, float py, float pz,
+                                               int& x, int& y, int& z) {
+  x = floor_to_int((px - c.lox) / c.voxel);
+  y = floor_to_int((py - c.loy) / c.voxel);
+  z = floor_to_int((pz - c.loz) / c.voxel);
+}
+
+// ---- neighbour iteration ----------------------------------------------------
+// Visits the reference's 27-cell stencil around (bx,by,bz) in its order
+// (dz, dy, dx from -1 to 1, z outermost) as 9 contiguous x-runs of sorted
+// slots.  A run of 3 cells whose total is <= max_scan cannot contain a cell
+// above MAX_CELL_SCAN, so it is visited whole; otherwise each cell is visited
+// separately with end = min(end, start + max_scan)
+// (shaders/sph_lambda.glsl:49-56).  off[c] is the first slot of cell c.
+template <int R, class F>
+__device__ __forceinline__ void for_each_run(const Grid& g, const int32_t* __restrict__ off,
+                                             int max_scan, int bx, int by, int bz, F&& f) {
+  const int x0 = max(bx - R, 0), x1 = min(bx + R, g.cx - 1);
+  for (int dz = -R; dz <= R; ++dz) {
+    const int z = bz + dz;
+    if (z < 0 || z >= g.cz) continue;
+    for (int dy = -R; dy <= R; ++dy) {
+      const int y = by + dy;
+      if (y < 0 || y >= g.cy) continue;
+      const int c0 = (z * g.cy + y) * g.cx + x0, c1 = c0 + (x1 - x0);
+      const int s = off[c0], e = off[c1 + 1];
+      if (e - s <= max_scan) {
+        f(s, e);
+      } else {
+        for (int c = c0; c <= c1; ++c) {
+          const int cs = off[c], ce = min(off[c + 1], cs + max_scan);
+          f(cs, ce);
+        }
+      }
+    }
+  }
+}
+
+// ---- launch helpers ---------------------------------------------------------
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(long long n, int block = kBlock) {
+  long long g = (n + block - 1) / block;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+// Workspace layout for the solver (offsets in bytes, 256-B aligned).
+struct Workspace {
+  size_t cnt, off, rank, key, sidx, skey, pred_s0, pred_s1, pos_s, vel_s, dl_s, partial, flags;
+  size_t total;
+};
+
+Workspace workspace_layout(const FxParams& p);
+
+// Host-side record of which workspaces hold a built grid, and for which
+// (n, n_cells): passes that read the previous grid (the normal pass, the
+// splat) refuse a workspace that never had one instead of reading garbage
+// offsets.
+void mark_grid(const void* work, int n, int n_cells);
+bool grid_ready(const void* work, int n, int n_cells);
+
+// Exclusive scan of int32 counts[0..n) into out[0..n] (out[n] = total).
+// partial must hold scan_partials(n) ints.  Launches 3 kernels.
+size_t scan_partials(long long n);
+int exclusive_scan_i32(const int32_t* counts, int32_t* out, long long n, int32_t* partial,
+                       hipStream_t st);
+
+}  // namespace fx

@@ -1,0 +1,118 @@
+// Device-wide exclusive scan of int32 counts (reduce -> scan partials ->
+// scan tiles).  Used for cell offsets (counting sort) and surface compaction.
+#include "fx_common.h"
+
+namespace fx {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+constexpr int kMaxPartials = 1 << 20;  // one block scans these
+
+size_t scan_partials(long long n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
+
+// Inclusive scan of one value per thread across a 256-thread block (4 waves
+// of 64), returns the inclusive prefix; *total receives the block sum.
+__device__ __forceinline__ int block_inclusive_scan(int v, int* lds, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  if (lane == 63) lds[wave] = v;
+  __syncthreads();
+  int add = 0;
+  for (int w = 0; w < wave; ++w) add += lds[w];
+  int tot = lds[0] + lds[1] + lds[2] + lds[3];
+  __syncthreads();
+  *total = tot;
+  return v + add;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int32_t* __restrict__ in,
+                                                               long long n,
+                                                               int32_t* __restrict__ partial) {
+  __shared__ int lds[4];
+  long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    long long q = base + i;
+    if (q < n) s += in[q];
+  }
+  int tot;
+  block_inclusive_scan(s, lds, &tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// Exclusive scan of partial[0..nb) in place by one block; partial[nb] = total.
+__global__ __launch_bounds__(1024) void k_scan_partials(int32_t* __restrict__ partial, int nb) {
+  __shared__ int lds[16];
+  const int per = (nb + 1023) / 1024;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int begin = threadIdx.x * per;
+  int s = 0;
+  for (int i = 0; i < per; ++i)
+    if (begin + i < nb) s += partial[begin + i];
+  int v = s;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  if (lane == 63) lds[wave] = v;
+  __syncthreads();
+  int add = 0;
+  for (int w = 0; w < wave; ++w) add += lds[w];
+  int run = v + add - s;  // exclusive prefix of this thread's chunk
+  for (int i = 0; i < per; ++i) {
+    if (begin + i < nb) {
+      int c = partial[begin + i];
+      partial[begin + i] = run;
+      run += c;
+    }
+  }
+  if (threadIdx.x == 1023) partial[nb] = v + add;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_final(const int32_t* __restrict__ in,
+                                                              int32_t* __restrict__ out,
+                                                              long long n,
+                                                              const int32_t* __restrict__ partial,
+                                                              int nb) {
+  __shared__ int lds[4];
+  long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    long long q = base + i;
+    v[i] = q < n ? in[q] : 0;
+    s += v[i];
+  }
+  int tot;
+  int incl = block_inclusive_scan(s, lds, &tot);
+  int run = partial[blockIdx.x] + incl - s;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    long long q = base + i;
+    if (q < n) out[q] = run;
+    run += v[i];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partial[nb];
+}
+
+int exclusive_scan_i32(const int32_t* counts, int32_t* out, long long n, int32_t* partial,
+                       hipStream_t st) {
+  long long nb = (n + kScanTile - 1) / kScanTile;
+  if (nb < 1) nb = 1;
+  if (nb > kMaxPartials) return fail(FX_ERANGE, "scan of %lld items exceeds %d tiles", n, kMaxPartials);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, st, counts, n, partial);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, partial, (int)nb);
+  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(kScanThreads), 0, st, counts, out, n,
+                     (const int32_t*)partial, (int)nb);
+  return check_launch("exclusive_scan_i32");
+}
+
+}  // namespace fx

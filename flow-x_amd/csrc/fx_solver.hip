@@ -1,0 +1,1060 @@
+// PBF solver kernels for gfx950 and the C ABI of the step / reset / grid path.
+//
+// Layout.  The caller's state is canonical (particle i at row i, AoS-4, the
+// reference texture layout).  Each grid build sorts particles by cell with a
+// deterministic counting sort (ascending cell, then ascending particle index)
+// and gathers them into "sorted space" inside the workspace, where slot t holds
+// particle sidx[t].  All neighbour passes run in sorted space: the particles of
+// one cell row are contiguous, so a 3-cell x-run of the reference's 27-cell
+// stencil (shaders/sph_lambda.glsl:39-70) is one contiguous slot range, loads
+// of neighbours are 16-byte float4 gathers from a few cache lines per wave,
+// and the neighbour-sum order is the reference's (dz, dy, dx, slot) order.
+// Results go back to canonical rows at the end of the substep.
+//
+// Exactness.  Streaming passes (predict, grid key, apply, velocity,
+// finalize) are compiled without FMA contraction and use IEEE division and
+// sqrt, so they match a float32 CPU evaluation of the GLSL bit for bit.  The
+// neighbour sums allow contraction and use v_rsq for 1/r; they match to a few
+// float32 ulps of each term.
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include <utility>
+
+#include "fx_common.h"
+
+#pragma clang fp contract(off)
+
+namespace fx {
+
+// ---- errors ----------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(FX_EHIP, "%s: %s", what, hipGetErrorString(e));
+  return FX_OK;
+}
+
+static std::mutex g_grid_mu;
+static std::unordered_map<const void*, std::pair<int, int>> g_grids;
+
+void mark_grid(const void* work, int n, int n_cells) {
+  std::lock_guard<std::mutex> lock(g_grid_mu);
+  g_grids[work] = std::make_pair(n, n_cells);
+}
+
+bool grid_ready(const void* work, int n, int n_cells) {
+  std::lock_guard<std::mutex> lock(g_grid_mu);
+  auto it = g_grids.find(work);
+  return it != g_grids.end() && it->second.first == n && it->second.second == n_cells;
+}
+
+// ---- constants (shaders/sph_common.glsl:68-91, sph_delta.glsl:44) --------
+float poly6_coef(float h) { return 315.0f / (64.0f * kPi * powf(h, 9.0f)); }
+
+Consts make_consts(const FxParams& p) {
+  Consts k;
+  k.h = p.h;
+  k.h2 = p.h * p.h;
+  k.mass = p.mass;
+  k.rho0 = p.rest_density;
+  k.eps = p.relaxation;
+  k.poly6 = poly6_coef(p.h);
+  k.spiky = -45.0f / (kPi * powf(p.h, 6.0f));
+  k.visc_lap = 45.0f / (kPi * powf(p.h, 6.0f));
+  float rq = powf(0.2f * p.h, 2.0f);
+  float wq = 0.0f;
+  if (rq < k.h2) {
+    float d = k.h2 - rq;
+    wq = k.poly6 * d * d * d;
+  }
+  k.w_q = wq;
+  k.scorr_k = p.scorr_k;
+  k.visc = p.viscosity;
+  k.dt = p.dt;
+  k.max_scan = p.max_cell_scan > 0 ? p.max_cell_scan : 512;
+  return k;
+}
+
+// ---- workspace -------------------------------------------------------------
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+Workspace workspace_layout(const FxParams& p) {
+  Workspace w;
+  size_t n = (size_t)(p.n > 0 ? p.n : 0), c = (size_t)(p.n_cells > 0 ? p.n_cells : 0);
+  size_t o = 0;
+  w.cnt = o;      o += align256((c + 1) * 4);
+  w.off = o;      o += align256((c + 1) * 4);
+  w.rank = o;     o += align256(n * 4 + 4);
+  w.key = o;      o += align256(n * 4 + 4);
+  w.sidx = o;     o += align256(n * 4 + 4);
+  w.skey = o;     o += align256(n * 4 + 4);
+  w.pred_s0 = o;  o += align256(n * 16 + 16);
+  w.pred_s1 = o;  o += align256(n * 16 + 16);
+  w.pos_s = o;    o += align256(n * 16 + 16);
+  w.vel_s = o;    o += align256(n * 16 + 16);
+  w.dl_s = o;     o += align256(n * 8 + 8);
+  w.partial = o;  o += align256(scan_partials((long long)c + 1) * 4 + 4);
+  w.flags = o;    o += align256(64 * 4);
+  w.total = o;
+  return w;
+}
+
+struct Ws {
+  int32_t *cnt, *off, *rank, *key, *sidx, *skey, *partial, *flags;
+  float4 *pred_s[2], *pos_s, *vel_s;
+  float2* dl_s;
+};
+
+static Ws bind_ws(const Workspace& w, void* base) {
+  char* b = (char*)base;
+  Ws s;
+  s.cnt = (int32_t*)(b + w.cnt);
+  s.off = (int32_t*)(b + w.off);
+  s.rank = (int32_t*)(b + w.rank);
+  s.key = (int32_t*)(b + w.key);
+  s.sidx = (int32_t*)(b + w.sidx);
+  s.skey = (int32_t*)(b + w.skey);
+  s.partial = (int32_t*)(b + w.partial);
+  s.flags = (int32_t*)(b + w.flags);
+  s.pred_s[0] = (float4*)(b + w.pred_s0);
+  s.pred_s[1] = (float4*)(b + w.pred_s1);
+  s.pos_s = (float4*)(b + w.pos_s);
+  s.vel_s = (float4*)(b + w.vel_s);
+  s.dl_s = (float2*)(b + w.dl_s);
+  return s;
+}
+
+// ---- streaming device functions (exact; contraction off at file scope) ----
+
+// shaders/sph_predict.glsl:23-57
+__device__ __forceinline__ void predict_one(const FxParams& p, float4 x, float4 v, float4 nrm,
+                                            float4& vo, float4& po) {
+  float dt = p.dt, sigma = p.surface_tension;
+  float vx = v.x, vy = v.y, vz = v.z;
+  vz += p.gravity * dt;
+  if (sigma > 0.0f) {
+    float mag = sqrtf(nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z);
+    if (mag > 1e-4f) {
+      float s = -sigma * nrm.w;
+      vx += s * nrm.x / mag / p.mass * dt;
+      vy += s * nrm.y / mag / p.mass * dt;
+      vz += s * nrm.z / mag / p.mass * dt;
+    }
+  }
+  float v_max = 0.4f * p.h / fmaxf(dt, 1e-6f);
+  float speed = sqrtf(vx * vx + vy * vy + vz * vz);
+  if (speed > v_max) {
+    float f = v_max / speed;
+    vx *= f;
+    vy *= f;
+    vz *= f;
+  }
+  po = make_float4(x.x + vx * dt, x.y + vy * dt, x.z + vz * dt, 1.0f);
+  vo = make_float4(vx, vy, vz, 0.0f);
+}
+
+// shaders/sph_finalize.glsl:15-96 (v already includes the XSPH delta)
+__device__ __forceinline__ void finalize_one(const FxParams& p, const Collider& col, float& px,
+                                             float& py, float& pz, float& vx, float& vy,
+                                             float& vz) {
+  const float radius = p.particle_radius, damping = p.boundary_damping;
+  if (col.occ) {
+    int cx, cy, cz;
+    collider_coord(col, px, py, pz, cx, cy, cz);
+    if (collider_occupied(col, cx, cy, cz)) {
+      const float voxel = col.voxel;
+      float nfx = 0.0f, nfy = 0.0f, nfz = 0.0f, best = 1e30f;
+      bool found = false;
+      for (int dz = -2; dz <= 2; ++dz)
+        for (int dy = -2; dy <= 2; ++dy)
+          for (int dx = -2; dx <= 2; ++dx) {
+            int x = cx + dx, y = cy + dy, z = cz + dz;
+            if (x < 0 || y < 0 || z < 0 || x >= col.kx || y >= col.ky || z >= col.kz) continue;
+            if (collider_occupied(col, x, y, z)) continue;
+            float ctx = col.lox + ((float)x + 0.5f) * voxel;
+            float cty = col.loy + ((float)y + 0.5f) * voxel;
+            float ctz = col.loz + ((float)z + 0.5f) * voxel;
+            float ex = ctx - px, ey = cty - py, ez = ctz - pz;
+            float d2 = ex * ex + ey * ey + ez * ez;
+            if (d2 < best) {
+              best = d2;
+              nfx = ctx;
+              nfy = cty;
+              nfz = ctz;
+              found = true;
+            }
+          }
+      if (found) {
+        float dist = sqrtf(best);
+        float ux, uy, uz;
+        if (dist > 1e-6f) {
+          ux = (nfx - px) / dist;
+          uy = (nfy - py) / dist;
+          uz = (nfz - pz) / dist;
+        } else {
+          ux = 0.0f;
+          uy = 0.0f;
+          uz = 1.0f;
+        }
+        float push = dist + radius;
+        px += ux * push;
+        py += uy * push;
+        pz += uz * push;
+        float vn = vx * ux + vy * uy + vz * uz;
+        if (vn < 0.0f) {
+          float f = vn * (1.0f + damping);
+          vx -= f * ux;
+          vy -= f * uy;
+          vz -= f * uz;
+        }
+      }
+    }
+  }
+  float lox = p.lo[0] + radius, loy = p.lo[1] + radius, loz = p.lo[2] + radius;
+  float hix = p.hi[0] - radius, hiy = p.hi[1] - radius, hiz = p.hi[2] - radius;
+  if (px < lox) {
+    px = lox;
+    vx = -vx * damping;
+  } else if (px > hix) {
+    px = hix;
+    vx = -vx * damping;
+  }
+  if (py < loy) {
+    py = loy;
+    vy = -vy * damping;
+  } else if (py > hiy) {
+    py = hiy;
+    vy = -vy * damping;
+  }
+  if (pz < loz) {
+    pz = loz;
+    vz = -vz * damping;
+  } else if (pz > hiz) {
+    pz = hiz;
+    vz = -vz * damping;
+  }
+}
+
+// ---- grid build kernels ---------------------------------------------------
+
+__global__ void k_fill_i32(int32_t* __restrict__ a, long long n, int32_t v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// Fused sph_predict + sph_grid_key + counting-sort histogram.  mode bit 0:
+// run predict (else pred is read as is); bit 1: count into cnt (rank[i] =
+// arrival order within the cell, canonicalised later).
+__global__ __launch_bounds__(256) void k_predict_key(FxParams p, Grid g, const float4* __restrict__ pos,
+                                                     float4* __restrict__ vel,
+                                                     const float4* __restrict__ nrm,
+                                                     float4* __restrict__ pred,
+                                                     int32_t* __restrict__ key,
+                                                     int32_t* __restrict__ cnt,
+                                                     int32_t* __restrict__ rank, int mode) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  float4 pp;
+  if (mode & 1) {
+    float4 vo;
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    predict_one(p, pos[i], vel[i], (p.surface_tension > 0.0f && nrm) ? nrm[i] : zero, vo, pp);
+    vel[i] = vo;
+    pred[i] = pp;
+  } else {
+    pp = pred[i];
+  }
+  if (mode & 6) {
+    int x, y, z;
+    cell_coord(g, pp.x, pp.y, pp.z, x, y, z);
+    const int k = cell_index(g, x, y, z);
+    key[i] = k;
+    if (mode & 2) rank[i] = atomicAdd(&cnt[k], 1);
+  }
+}
+
+// Counting-sort histogram of given keys (the FX_PASS_SORT entry).
+// Keys outside [0, nc) are clamped into range (and flagged) so that a bad
+// input cannot index out of bounds.
+__global__ void k_count_keys(int n, int nc, int32_t* __restrict__ key, int32_t* __restrict__ cnt,
+                             int32_t* __restrict__ rank, int32_t* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int k = key[i];
+  if (k < 0 || k >= nc) {
+    atomicOr(bad, 1);
+    k = min(max(k, 0), nc - 1);
+    key[i] = k;
+  }
+  rank[i] = atomicAdd(&cnt[k], 1);
+}
+
+// Cell ranges in the reference format from the offsets (-1 for empty cells).
+__global__ void k_cell_ranges(int nc, const int32_t* __restrict__ off, int32_t* __restrict__ start,
+                              int32_t* __restrict__ end) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int s = off[c], e = off[c + 1];
+  start[c] = e > s ? s : -1;
+  end[c] = e > s ? e : -1;
+}
+
+__global__ void k_scatter(int n, const int32_t* __restrict__ key, const int32_t* __restrict__ rank,
+                          const int32_t* __restrict__ off, int32_t* __restrict__ sidx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = key[i];
+  sidx[off[k] + rank[i]] = i;
+}
+
+// Canonicalise each cell's run to ascending particle index.  Runs up to
+// kSmallCell are insertion-sorted by their own thread; longer runs are queued
+// for k_sort_big_cells.
+constexpr int kSmallCell = 128;
+
+__global__ void k_canon_cells(int nc, const int32_t* __restrict__ off, int32_t* __restrict__ sidx,
+                              int32_t* __restrict__ big_list, int32_t* __restrict__ big_count) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int s = off[c], e = off[c + 1], m = e - s;
+  if (m <= 1) return;
+  if (m > kSmallCell) {
+    big_list[atomicAdd(big_count, 1)] = c;
+    return;
+  }
+  for (int a = s + 1; a < e; ++a) {
+    const int v = sidx[a];
+    int b = a - 1;
+    while (b >= s && sidx[b] > v) {
+      sidx[b + 1] = sidx[b];
+      --b;
+    }
+    sidx[b + 1] = v;
+  }
+}
+
+// Block-cooperative sort of long runs: a bitonic network in which every
+// compare-exchange puts the minimum at the lower slot (first step of each
+// merge compares i with i ^ (k-1)), so slots past the run end act as +inf
+// and are never touched.  Grid-stride over the queued cells.
+__global__ __launch_bounds__(256) void k_sort_big_cells(const int32_t* __restrict__ off,
+                                                        int32_t* __restrict__ sidx,
+                                                        const int32_t* __restrict__ big_list,
+                                                        const int32_t* __restrict__ big_count) {
+  const int count = *big_count;
+  for (int b = blockIdx.x; b < count; b += gridDim.x) {
+    const int c = big_list[b];
+    const int s = off[c], m = off[c + 1] - s;
+    int P = 1;
+    while (P < m) P <<= 1;
+    int32_t* a = sidx + s;
+    for (int k = 2; k <= P; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+          const int partner = (j == (k >> 1)) ? (i ^ (k - 1)) : (i ^ j);
+          if (partner > i && partner < m) {
+            const int x = a[i], y = a[partner];
+            if (x > y) {
+              a[i] = y;
+              a[partner] = x;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Gather canonical state into sorted space.  pred_s.w carries the
+// neighbour's density for the normal pass (lambda4.x, or 0 after a reset:
+// the reference's zeroed lambda_img, solver/sph.py:716-724).
+__global__ void k_gather(int n, const int32_t* __restrict__ sidx, const int32_t* __restrict__ key,
+                         const float4* __restrict__ pred, const float4* __restrict__ pos,
+                         const float4* __restrict__ lambda4, float4* __restrict__ pred_s,
+                         float4* __restrict__ pos_s, float2* __restrict__ dl_s,
+                         int32_t* __restrict__ skey_out, int32_t* __restrict__ skey_int,
+                         int32_t* __restrict__ sidx_out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = sidx[t];
+  float4 q = pred[i];
+  float2 dl = make_float2(0.0f, 0.0f);
+  if (lambda4) {
+    const float4 l = lambda4[i];
+    dl = make_float2(l.x, l.y);
+  }
+  q.w = dl.x;
+  pred_s[t] = q;
+  if (pos_s) pos_s[t] = pos[i];
+  if (dl_s) dl_s[t] = dl;
+  if (key) {
+    const int k = key[i];
+    if (skey_int) skey_int[t] = k;
+    if (skey_out) skey_out[t] = k;
+  }
+  if (sidx_out) sidx_out[t] = i;
+}
+
+// Offsets from sorted keys: off[c] = first slot with key >= c.
+__global__ void k_off_from_sorted(int n, int nc, const int32_t* __restrict__ skey,
+                                  int32_t* __restrict__ off) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > n) return;
+  const int prev = t == 0 ? -1 : min(skey[t - 1], nc);
+  const int cur = t == n ? nc : min(skey[t], nc);
+  for (int c = prev + 1; c <= cur; ++c) off[c] = t;
+}
+
+// The reference's own cell-range pass (shaders/sph_cell_range.glsl:10-32)
+// over the n sorted slots: start where the key changes from the previous
+// slot, end where it changes to the next.
+__global__ void k_cell_range_ref(int n, int nc, const int32_t* __restrict__ skey,
+                                 int32_t* __restrict__ start, int32_t* __restrict__ end) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int key = skey[i];
+  if (key < 0 || key >= nc) return;
+  const int prev = (i == 0) ? -1 : skey[i - 1];
+  const int next = (i == n - 1) ? -1 : skey[i + 1];
+  if (key != prev) start[key] = i;
+  if (key != next) end[key] = i + 1;
+}
+
+// ---- neighbour passes (sorted space) -------------------------------------
+
+// shaders/sph_lambda.glsl:17-77.  Writes dl_s[t] = (max(rho,1e-6), lambda),
+// pred_s[t].w = lambda (read by the delta pass with the position), and the
+// canonical lambda4 row if requested.
+__global__ __launch_bounds__(256) void k_lambda(int n, Grid g, Consts k,
+                                                float4* __restrict__ pred_s,
+                                                const int32_t* __restrict__ off,
+                                                float2* __restrict__ dl_s,
+                                                const int32_t* __restrict__ sidx,
+                                                float4* __restrict__ lambda4) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float4 pi = pred_s[t];
+  int bx, by, bz;
+  cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
+  float dens = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f, gn2 = 0.0f;
+  {
+#pragma clang fp contract(fast)
+    const float h = k.h, h2 = k.h2;
+    for_each_run<1>(g, off, k.max_scan, bx, by, bz, [&](int s, int e) {
+      for (int q = s; q < e; ++q) {
+        const float4 pj = pred_s[q];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < h2) {
+          const float d = h2 - r2;
+          dens += k.poly6 * d * d * d;
+          if (q != t && r2 > 1e-12f) {
+            const float inv_r = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * inv_r;
+            const float w = h - r;
+            const float c = k.spiky * w * w * inv_r;
+            const float ax = c * dx, ay = c * dy, az = c * dz;
+            gx += ax;
+            gy += ay;
+            gz += az;
+            gn2 += ax * ax + ay * ay + az * az;
+          }
+        }
+      }
+    });
+  }
+  // mass factors hoisted out of the sums: density = mass * sum W,
+  // grad = mass * sum(...)
+  dens = k.mass * dens;
+  gx *= k.mass;
+  gy *= k.mass;
+  gz *= k.mass;
+  gn2 *= k.mass * k.mass;
+  const float constraint = fmaxf(dens, 1e-6f) / k.rho0 - 1.0f;
+  const float denom = (gx * gx + gy * gy + gz * gz + gn2) / (k.rho0 * k.rho0) + k.eps;
+  const float lambda = -constraint / fmaxf(denom, 1e-9f);
+  const float rho = fmaxf(dens, 1e-6f);
+  dl_s[t] = make_float2(rho, lambda);
+  pred_s[t].w = lambda;
+  if (lambda4) lambda4[sidx[t]] = make_float4(rho, lambda, 0.0f, 0.0f);
+}
+
+// shaders/sph_delta.glsl:22-85 fused with shaders/sph_apply_delta.glsl:10-20
+// (Jacobi ping-pong: reads pred_in, writes pred_out) and, on the last
+// iteration, shaders/sph_velocity.glsl:12-24 into vel_s (w = own density).
+// delta4 (canonical, optional) receives the correction itself.
+__global__ __launch_bounds__(256) void k_delta(int n, Grid g, Consts k,
+                                               const float4* __restrict__ pred_in,
+                                               const float2* __restrict__ dl_s,
+                                               const int32_t* __restrict__ off,
+                                               float4* __restrict__ pred_out,
+                                               const float4* __restrict__ pos_s,
+                                               float4* __restrict__ vel_s,
+                                               const int32_t* __restrict__ sidx,
+                                               float4* __restrict__ delta4) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float4 pi = pred_in[t];
+  const float li = pi.w;
+  int bx, by, bz;
+  cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+  {
+#pragma clang fp contract(fast)
+    const float h = k.h, h2 = k.h2;
+    const bool scorr = k.scorr_k > 0.0f && k.w_q > 1e-9f;
+    const float inv_wq = scorr ? 1.0f / k.w_q : 0.0f;
+    for_each_run<1>(g, off, k.max_scan, bx, by, bz, [&](int s, int e) {
+      for (int q = s; q < e; ++q) {
+        const float4 pj = pred_in[q];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < h2 && r2 > 1e-12f && q != t) {
+          const float inv_r = __builtin_amdgcn_rsqf(r2);
+          const float r = r2 * inv_r;
+          float sc = 0.0f;
+          if (scorr) {
+            const float d = h2 - r2;
+            const float ratio = k.poly6 * d * d * d * inv_wq;
+            const float r4 = ratio * ratio;
+            sc = -k.scorr_k * (r4 * r4);
+          }
+          const float w = h - r;
+          const float c = (li + pj.w + sc) * (k.spiky * w * w) * inv_r;
+          ox += c * dx;
+          oy += c * dy;
+          oz += c * dz;
+        }
+      }
+    });
+  }
+  const float ddx = ox / k.rho0, ddy = oy / k.rho0, ddz = oz / k.rho0;
+  const float2 dl = dl_s[t];
+  const float4 pn = make_float4(pi.x + ddx, pi.y + ddy, pi.z + ddz, dl.x);
+  if (pred_out) pred_out[t] = pn;
+  if (vel_s) {
+    const float dtc = fmaxf(k.dt, 1e-6f);
+    const float4 po = pos_s[t];
+    vel_s[t] = make_float4((pn.x - po.x) / dtc, (pn.y - po.y) / dtc, (pn.z - po.z) / dtc, dl.x);
+  }
+  if (delta4) delta4[sidx[t]] = make_float4(ddx, ddy, ddz, 0.0f);
+}
+
+// shaders/sph_velocity.glsl:12-24 in sorted space (used when iterations = 0).
+__global__ void k_velocity_s(int n, float dt, const float4* __restrict__ pred_s,
+                             const float4* __restrict__ pos_s, const float2* __restrict__ dl_s,
+                             float4* __restrict__ vel_s) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float dtc = fmaxf(dt, 1e-6f);
+  const float4 pn = pred_s[t], po = pos_s[t];
+  vel_s[t] = make_float4((pn.x - po.x) / dtc, (pn.y - po.y) / dtc, (pn.z - po.z) / dtc, dl_s[t].x);
+}
+
+// shaders/sph_xsph.glsl:28-82 and, fused, shaders/sph_finalize.glsl:15-96.
+// pred_s: final predicted positions; vel_s: (raw velocity, density).
+// FUSE: apply the correction, collide, clamp and write canonical pos / vel
+// (plus pos_s for the splat); otherwise write the correction to delta4 only.
+template <bool FUSE>
+__global__ __launch_bounds__(256) void k_xsph(int n, FxParams p, Grid g, Consts k,
+                                              Collider col, const float4* __restrict__ pred_s,
+                                              const float4* __restrict__ vel_s,
+                                              const int32_t* __restrict__ off,
+                                              const int32_t* __restrict__ sidx,
+                                              float4* __restrict__ delta4,
+                                              float4* __restrict__ pos4, float4* __restrict__ vel4,
+                                              float4* __restrict__ pos_s,
+                                              float4* __restrict__ pred4_out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float4 pi = pred_s[t];
+  const float4 vi = vel_s[t];
+  int bx, by, bz;
+  cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  {
+#pragma clang fp contract(fast)
+    const float h2 = k.h2;
+    for_each_run<1>(g, off, k.max_scan, bx, by, bz, [&](int s, int e) {
+      for (int q = s; q < e; ++q) {
+        const float4 pj = pred_s[q];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < h2 && q != t) {
+          const float4 vj = vel_s[q];
+          const float d = h2 - r2;
+          const float w = d * d * d * __builtin_amdgcn_rcpf(fmaxf(vj.w, 1e-6f));
+          ax += (vj.x - vi.x) * w;
+          ay += (vj.y - vi.y) * w;
+          az += (vj.z - vi.z) * w;
+        }
+      }
+    });
+  }
+  const float f = k.visc * (k.mass * k.poly6);
+  const float dvx = f * ax, dvy = f * ay, dvz = f * az;
+  const int i = sidx[t];
+  if (!FUSE) {
+    delta4[i] = make_float4(dvx, dvy, dvz, 0.0f);
+    return;
+  }
+  if (delta4) delta4[i] = make_float4(dvx, dvy, dvz, 0.0f);
+  float vx = vi.x + dvx, vy = vi.y + dvy, vz = vi.z + dvz;
+  float px = pi.x, py = pi.y, pz = pi.z;
+  if (pred4_out) pred4_out[i] = make_float4(px, py, pz, 1.0f);
+  finalize_one(p, col, px, py, pz, vx, vy, vz);
+  const float4 pf = make_float4(px, py, pz, 1.0f);
+  pos4[i] = pf;
+  vel4[i] = make_float4(vx, vy, vz, 0.0f);
+  if (pos_s) pos_s[t] = pf;
+}
+
+// shaders/sph_normal.glsl:24-81 over the previous grid; pred_s.w = the
+// neighbour's density from the last lambda pass.  Writes canonical normal4.
+__global__ __launch_bounds__(256) void k_normal(int n, Grid g, Consts k,
+                                                const float4* __restrict__ pred_s,
+                                                const int32_t* __restrict__ off,
+                                                const int32_t* __restrict__ sidx,
+                                                float4* __restrict__ normal4) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float4 pi = pred_s[t];
+  int bx, by, bz;
+  cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f, lap = 0.0f;
+  {
+#pragma clang fp contract(fast)
+    const float h = k.h, h2 = k.h2;
+    for_each_run<1>(g, off, k.max_scan, bx, by, bz, [&](int s, int e) {
+      for (int q = s; q < e; ++q) {
+        const float4 pj = pred_s[q];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < h2 && q != t) {
+          const float inv_rho = __builtin_amdgcn_rcpf(fmaxf(pj.w, 1e-6f));
+          const float tt = h2 - r2;
+          const float gsc = tt * tt * inv_rho;
+          nx += gsc * dx;
+          ny += gsc * dy;
+          nz += gsc * dz;
+          const float r = r2 * __builtin_amdgcn_rsqf(r2);
+          lap += (h - r) * inv_rho;
+        }
+      }
+    });
+  }
+  const float gfac = k.mass * (k.poly6 * -6.0f);
+  nx *= gfac;
+  ny *= gfac;
+  nz *= gfac;
+  lap *= k.mass * k.visc_lap;
+  const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
+  const float curvature = (mag > 1e-4f) ? -lap / mag : 0.0f;
+  normal4[sidx[t]] = make_float4(nx, ny, nz, curvature);
+}
+
+// ---- canonical streaming passes (FX_PASS_*) --------------------------------
+
+__global__ void k_apply_delta_c(int n, float4* __restrict__ pred, const float4* __restrict__ delta) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = pred[i], d = delta[i];
+  pred[i] = make_float4(p.x + d.x, p.y + d.y, p.z + d.z, 1.0f);
+}
+
+__global__ void k_velocity_c(int n, float dt, const float4* __restrict__ pred,
+                             const float4* __restrict__ pos, float4* __restrict__ vel) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float dtc = fmaxf(dt, 1e-6f);
+  const float4 a = pred[i], b = pos[i];
+  vel[i] = make_float4((a.x - b.x) / dtc, (a.y - b.y) / dtc, (a.z - b.z) / dtc, 0.0f);
+}
+
+__global__ void k_finalize_c(FxParams p, Collider col, float4* __restrict__ vel,
+                             const float4* __restrict__ delta, const float4* __restrict__ pred,
+                             float4* __restrict__ pos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  const float4 v = vel[i], d = delta[i], q = pred[i];
+  float vx = v.x + d.x, vy = v.y + d.y, vz = v.z + d.z;
+  float px = q.x, py = q.y, pz = q.z;
+  finalize_one(p, col, px, py, pz, vx, vy, vz);
+  pos[i] = make_float4(px, py, pz, 1.0f);
+  vel[i] = make_float4(vx, vy, vz, 0.0f);
+}
+
+// reset / cache load: pred = pos, w lanes as the reference stores them
+__global__ void k_load_state(int n, float4* __restrict__ pos, float4* __restrict__ vel,
+                             float4* __restrict__ pred, float4* __restrict__ lambda4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = pos[i];
+  p.w = 1.0f;
+  pos[i] = p;
+  float4 v = vel[i];
+  v.w = 0.0f;
+  vel[i] = v;
+  if (pred) pred[i] = p;
+  if (lambda4) lambda4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+
+__global__ void k_w_from_lambda(int n, float4* pred_s, const float2* dl_s) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  pred_s[t].w = dl_s[t].y;
+}
+
+__global__ void k_gather_xsph(int n, const int32_t* sidx, const float4* pred, const float4* vel,
+                              const float4* lambda4, float4* pred_s, float4* vel_s) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = sidx[t];
+  pred_s[t] = pred[i];
+  float4 v = vel[i];
+  v.w = lambda4[i].x;
+  vel_s[t] = v;
+}
+
+__global__ void k_sorted_keys(int n, const int32_t* __restrict__ sidx, const int32_t* __restrict__ key,
+                              int32_t* __restrict__ skey) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  skey[t] = key[sidx[t]];
+}
+
+// ---- host side --------------------------------------------------------------
+
+static int validate(const FxParams* p, const FxState* s, bool need_work) {
+  if (!p || !s) return fail(FX_EINVAL, "null params or state");
+  if (p->n < 0) return fail(FX_EINVAL, "particle count %d < 0", p->n);
+  if (p->cells[0] < 1 || p->cells[1] < 1 || p->cells[2] < 1)
+    return fail(FX_EINVAL, "cell dims (%d,%d,%d) must be >= 1", p->cells[0], p->cells[1], p->cells[2]);
+  long long nc = (long long)p->cells[0] * p->cells[1] * p->cells[2];
+  if (nc != p->n_cells) return fail(FX_EINVAL, "n_cells %d != dims product %lld", p->n_cells, nc);
+  if (nc >= (1LL << 30)) return fail(FX_ERANGE, "cell count %lld too large", nc);
+  if (!(p->cell_size > 0.0f)) return fail(FX_EINVAL, "cell_size must be > 0");
+  if (!(p->h > 0.0f)) return fail(FX_EINVAL, "smoothing radius must be > 0");
+  if (p->iterations < 0) return fail(FX_EINVAL, "iterations < 0");
+  if (p->collider_voxel > 0.0f &&
+      (p->collider_dims[0] < 1 || p->collider_dims[1] < 1 || p->collider_dims[2] < 1))
+    return fail(FX_EINVAL, "collider dims must be >= 1 when collider_voxel > 0");
+  if (need_work) {
+    size_t need = workspace_layout(*p).total;
+    if (!s->work) return fail(FX_EINVAL, "workspace pointer is NULL");
+    if (s->work_bytes < need)
+      return fail(FX_ENOMEM, "workspace too small: %zu < %zu bytes", s->work_bytes, need);
+  }
+  return FX_OK;
+}
+
+#define LAUNCH(kernel, n, st, ...) \
+  hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(kBlock), 0, st, __VA_ARGS__)
+
+static int fill(int32_t* a, long long n, int32_t v, hipStream_t st) {
+  if (n <= 0) return FX_OK;
+  LAUNCH(k_fill_i32, n, st, a, n, v);
+  return FX_OK;
+}
+
+// Counting sort of ws.key (counts already in ws.cnt, ranks in ws.rank) into
+// ws.sidx: scan, scatter, canonicalise.  Writes reference-format ranges if
+// requested.
+static int sort_from_counts(const FxParams& p, const FxState& s, const Ws& w, hipStream_t st) {
+  const int n = p.n, nc = p.n_cells;
+  FX_TRY(exclusive_scan_i32(w.cnt, w.off, nc, w.partial, st));
+  if (s.cell_start && s.cell_end) LAUNCH(k_cell_ranges, nc, st, nc, w.off, s.cell_start, s.cell_end);
+  if (n > 0) {
+    LAUNCH(k_scatter, n, st, n, w.key, w.rank, w.off, w.sidx);
+    // big-cell queue reuses the rank array (dead after the scatter)
+    FX_TRY(fill(w.flags, 1, 0, st));
+    LAUNCH(k_canon_cells, nc, st, nc, w.off, w.sidx, w.rank, w.flags);
+    hipLaunchKernelGGL(k_sort_big_cells, dim3(64), dim3(256), 0, st, w.off, w.sidx, w.rank, w.flags);
+  }
+  return check_launch("sort_from_counts");
+}
+
+// Full grid build from s.pred (canonical) with optional fused predict.
+// target: which pred_s buffer receives the gathered predicted positions.
+static int build_grid(const FxParams& p, FxState& s, const Ws& w, bool do_predict, int target,
+                      hipStream_t st) {
+  const int n = p.n, nc = p.n_cells;
+  Grid g = make_grid(p);
+  FX_TRY(fill(w.cnt, nc + 1, 0, st));
+  int32_t* key = w.key;
+  if (n > 0) {
+    LAUNCH(k_predict_key, n, st, p, g, (const float4*)s.pos, (float4*)s.vel,
+           (const float4*)s.normal, (float4*)s.pred, key, w.cnt, w.rank, do_predict ? 3 : 2);
+  }
+  FX_TRY(sort_from_counts(p, s, w, st));
+  if (n > 0) {
+    LAUNCH(k_gather, n, st, n, w.sidx, key, (const float4*)s.pred, (const float4*)s.pos,
+           (const float4*)nullptr, w.pred_s[target], w.pos_s, w.dl_s, s.sorted_key, w.skey,
+           s.sorted_idx);
+    if (s.key) FX_HIP_CHECK(hipMemcpyAsync(s.key, key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  }
+  FX_TRY(check_launch("build_grid"));
+  mark_grid(s.work, n, nc);
+  return FX_OK;
+}
+
+static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
+  const int n = p.n;
+  Grid g = make_grid(p);
+  Consts k = make_consts(p);
+  Collider col = make_collider(p, s.collider);
+  // sph_normal runs first, over the previous substep's grid and state
+  // (shaders/sph_normal.glsl:15-21); its output feeds sph_predict.
+  if (p.surface_tension > 0.0f) {
+    if (!s.normal) return fail(FX_EINVAL, "surface_tension > 0 needs a normal buffer");
+    if (!grid_ready(s.work, n, p.n_cells))
+      return fail(FX_EINVAL, "surface tension reads the previous grid: call fx_load_state first");
+    if (n > 0) LAUNCH(k_normal, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal);
+  }
+  const int iters = p.iterations;
+  int cur = iters & 1;  // the last iteration writes buffer 0
+  FX_TRY(build_grid(p, s, w, true, cur, st));
+  if (n == 0) return check_launch("substep");
+  for (int it = 0; it < iters; ++it) {
+    const bool last = it == iters - 1;
+    LAUNCH(k_lambda, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx,
+           (last && s.lambda) ? (float4*)s.lambda : (float4*)nullptr);
+    LAUNCH(k_delta, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
+           w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
+           (float4*)nullptr);
+    cur ^= 1;
+  }
+  if (iters == 0) LAUNCH(k_velocity_s, n, st, n, p.dt, w.pred_s[0], w.pos_s, w.dl_s, w.vel_s);
+  LAUNCH(k_xsph<true>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+         w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred);
+  return check_launch("substep");
+}
+
+// Prepare sorted-space inputs for a canonical neighbour pass: offsets from
+// s.sorted_key, the permutation from s.sorted_idx.
+static int canonical_grid(const FxParams& p, const FxState& s, const Ws& w, hipStream_t st) {
+  if (!s.sorted_key || !s.sorted_idx) return fail(FX_EINVAL, "pass needs sorted_key and sorted_idx");
+  const int n = p.n, nc = p.n_cells;
+  if (n > 0) {
+    FX_HIP_CHECK(hipMemcpyAsync(w.sidx, s.sorted_idx, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    FX_HIP_CHECK(hipMemcpyAsync(w.skey, s.sorted_key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  }
+  hipLaunchKernelGGL(k_off_from_sorted, dim3(grid_for((long long)n + 1)), dim3(kBlock), 0, st, n, nc,
+                     w.skey, w.off);
+  FX_TRY(check_launch("canonical_grid"));
+  mark_grid(s.work, n, nc);
+  return FX_OK;
+}
+
+static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
+  const int n = p.n, nc = p.n_cells;
+  Grid g = make_grid(p);
+  Consts k = make_consts(p);
+  Collider col = make_collider(p, s.collider);
+  auto need = [&](const void* ptr, const char* name) -> int {
+    if (!ptr) return fail(FX_EINVAL, "pass %d needs the %s buffer", id, name);
+    return FX_OK;
+  };
+  switch (id) {
+    case FX_PASS_PREDICT:
+      FX_TRY(need(s.pos, "pos"));
+      FX_TRY(need(s.vel, "vel"));
+      FX_TRY(need(s.pred, "pred"));
+      if (p.surface_tension > 0.0f) FX_TRY(need(s.normal, "normal"));
+      if (n > 0)
+        LAUNCH(k_predict_key, n, st, p, g, (const float4*)s.pos, (float4*)s.vel,
+               (const float4*)s.normal, (float4*)s.pred, (int32_t*)nullptr, (int32_t*)nullptr,
+               (int32_t*)nullptr, 1);
+      break;
+    case FX_PASS_GRID_KEY:
+      FX_TRY(need(s.pred, "pred"));
+      FX_TRY(need(s.key, "key"));
+      if (n > 0)
+        LAUNCH(k_predict_key, n, st, p, g, (const float4*)nullptr, (float4*)nullptr,
+               (const float4*)nullptr, (float4*)s.pred, s.key, (int32_t*)nullptr, (int32_t*)nullptr,
+               4);
+      break;
+    case FX_PASS_SORT: {
+      FX_TRY(need(s.key, "key"));
+      FX_TRY(need(s.sorted_key, "sorted_key"));
+      FX_TRY(need(s.sorted_idx, "sorted_idx"));
+      FX_TRY(fill(w.cnt, nc + 1, 0, st));
+      FX_TRY(fill(w.flags + 1, 1, 0, st));
+      if (n > 0) {
+        FX_HIP_CHECK(hipMemcpyAsync(w.key, s.key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+        LAUNCH(k_count_keys, n, st, n, nc, w.key, w.cnt, w.rank, w.flags + 1);
+      }
+      FxState t = s;
+      t.cell_start = t.cell_end = nullptr;
+      FX_TRY(sort_from_counts(p, t, w, st));
+      mark_grid(s.work, n, nc);
+      if (n > 0) {
+        FX_HIP_CHECK(hipMemcpyAsync(s.sorted_idx, w.sidx, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+        LAUNCH(k_sorted_keys, n, st, n, w.sidx, w.key, s.sorted_key);
+      }
+      break;
+    }
+    case FX_PASS_CELL_CLEAR:
+      FX_TRY(need(s.cell_start, "cell_start"));
+      FX_TRY(need(s.cell_end, "cell_end"));
+      FX_TRY(fill(s.cell_start, nc, -1, st));
+      FX_TRY(fill(s.cell_end, nc, -1, st));
+      break;
+    case FX_PASS_CELL_RANGE:
+      FX_TRY(need(s.sorted_key, "sorted_key"));
+      FX_TRY(need(s.cell_start, "cell_start"));
+      FX_TRY(need(s.cell_end, "cell_end"));
+      if (n > 0) LAUNCH(k_cell_range_ref, n, st, n, nc, s.sorted_key, s.cell_start, s.cell_end);
+      break;
+    case FX_PASS_LAMBDA:
+      FX_TRY(need(s.pred, "pred"));
+      FX_TRY(need(s.lambda, "lambda"));
+      FX_TRY(canonical_grid(p, s, w, st));
+      if (n > 0) {
+        LAUNCH(k_gather, n, st, n, w.sidx, (const int32_t*)nullptr, (const float4*)s.pred, (const float4*)nullptr,
+               (const float4*)nullptr, w.pred_s[0], (float4*)nullptr, (float2*)nullptr,
+               (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
+        LAUNCH(k_lambda, n, st, n, g, k, w.pred_s[0], w.off, w.dl_s, w.sidx, (float4*)s.lambda);
+      }
+      break;
+    case FX_PASS_DELTA:
+      FX_TRY(need(s.pred, "pred"));
+      FX_TRY(need(s.lambda, "lambda"));
+      FX_TRY(need(s.delta, "delta"));
+      FX_TRY(canonical_grid(p, s, w, st));
+      if (n > 0) {
+        // pred_s.w must hold lambda_j: gather (rho, lambda) then move lambda to w
+        LAUNCH(k_gather, n, st, n, w.sidx, (const int32_t*)nullptr, (const float4*)s.pred, (const float4*)nullptr,
+               (const float4*)s.lambda, w.pred_s[0], (float4*)nullptr, w.dl_s, (int32_t*)nullptr,
+               (int32_t*)nullptr, (int32_t*)nullptr);
+        LAUNCH(k_w_from_lambda, n, st, n, w.pred_s[0], (const float2*)w.dl_s);
+        LAUNCH(k_delta, n, st, n, g, k, (const float4*)w.pred_s[0], (const float2*)w.dl_s, w.off,
+               (float4*)nullptr, (const float4*)nullptr, (float4*)nullptr, w.sidx, (float4*)s.delta);
+      }
+      break;
+    case FX_PASS_APPLY_DELTA:
+      FX_TRY(need(s.pred, "pred"));
+      FX_TRY(need(s.delta, "delta"));
+      if (n > 0) LAUNCH(k_apply_delta_c, n, st, n, (float4*)s.pred, (const float4*)s.delta);
+      break;
+    case FX_PASS_VELOCITY:
+      FX_TRY(need(s.pred, "pred"));
+      FX_TRY(need(s.pos, "pos"));
+      FX_TRY(need(s.vel, "vel"));
+      if (n > 0) LAUNCH(k_velocity_c, n, st, n, p.dt, (const float4*)s.pred, (const float4*)s.pos, (float4*)s.vel);
+      break;
+    case FX_PASS_XSPH:
+      FX_TRY(need(s.pred, "pred"));
+      FX_TRY(need(s.vel, "vel"));
+      FX_TRY(need(s.lambda, "lambda"));
+      FX_TRY(need(s.delta, "delta"));
+      FX_TRY(canonical_grid(p, s, w, st));
+      if (n > 0) {
+        LAUNCH(k_gather_xsph, n, st, n, w.sidx, (const float4*)s.pred, (const float4*)s.vel,
+               (const float4*)s.lambda, w.pred_s[0], w.vel_s);
+        LAUNCH(k_xsph<false>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
+               (const float4*)w.vel_s, w.off, w.sidx, (float4*)s.delta, (float4*)nullptr,
+               (float4*)nullptr, (float4*)nullptr, (float4*)nullptr);
+      }
+      break;
+    case FX_PASS_FINALIZE:
+      FX_TRY(need(s.pos, "pos"));
+      FX_TRY(need(s.vel, "vel"));
+      FX_TRY(need(s.delta, "delta"));
+      FX_TRY(need(s.pred, "pred"));
+      if (n > 0)
+        LAUNCH(k_finalize_c, n, st, p, col, (float4*)s.vel, (const float4*)s.delta,
+               (const float4*)s.pred, (float4*)s.pos);
+      break;
+    case FX_PASS_NORMAL:
+      FX_TRY(need(s.pred, "pred"));
+      FX_TRY(need(s.lambda, "lambda"));
+      FX_TRY(need(s.normal, "normal"));
+      FX_TRY(canonical_grid(p, s, w, st));
+      if (n > 0) {
+        LAUNCH(k_gather, n, st, n, w.sidx, (const int32_t*)nullptr, (const float4*)s.pred, (const float4*)nullptr,
+               (const float4*)s.lambda, w.pred_s[0], (float4*)nullptr, (float2*)nullptr,
+               (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
+        LAUNCH(k_normal, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal);
+      }
+      break;
+    default:
+      return fail(FX_EINVAL, "unknown pass id %d", id);
+  }
+  return check_launch("fx_pass");
+}
+
+}  // namespace fx
+
+// ---- C ABI -------------------------------------------------------------------
+using namespace fx;
+
+extern "C" {
+
+const char* fx_last_error(void) { return g_last_error.c_str(); }
+
+int fx_abi_version(void) { return FX_ABI_VERSION; }
+
+size_t fx_workspace_bytes(const FxParams* p) {
+  if (!p) return 0;
+  return workspace_layout(*p).total;
+}
+
+int fx_pass(int pass_id, const FxParams* p, FxState* s, void* stream) {
+  FX_TRY(validate(p, s, true));
+  Ws w = bind_ws(workspace_layout(*p), s->work);
+  return run_pass(pass_id, *p, *s, w, (hipStream_t)stream);
+}
+
+int fx_build_grid(const FxParams* p, FxState* s, void* stream) {
+  FX_TRY(validate(p, s, true));
+  if (p->n > 0 && (!s->pred || !s->pos)) return fail(FX_EINVAL, "build_grid needs pred and pos");
+  Ws w = bind_ws(workspace_layout(*p), s->work);
+  return build_grid(*p, *s, w, false, 0, (hipStream_t)stream);
+}
+
+int fx_substep(const FxParams* p, FxState* s, void* stream) {
+  FX_TRY(validate(p, s, true));
+  if (p->n > 0 && (!s->pos || !s->vel)) return fail(FX_EINVAL, "substep needs pos and vel");
+  if (p->n > 0 && !s->pred) return fail(FX_EINVAL, "substep needs a pred buffer");
+  Ws w = bind_ws(workspace_layout(*p), s->work);
+  return substep(*p, *s, w, (hipStream_t)stream);
+}
+
+int fx_step(const FxParams* p, FxState* s, int substeps, void* stream) {
+  if (substeps < 0) return fail(FX_EINVAL, "substeps < 0");
+  for (int i = 0; i < substeps; ++i) FX_TRY(fx_substep(p, s, stream));
+  return FX_OK;
+}
+
+int fx_load_state(const FxParams* p, FxState* s, void* stream) {
+  FX_TRY(validate(p, s, true));
+  if (p->n > 0 && (!s->pos || !s->vel || !s->pred))
+    return fail(FX_EINVAL, "load_state needs pos, vel and pred");
+  hipStream_t st = (hipStream_t)stream;
+  if (p->n > 0)
+    LAUNCH(k_load_state, p->n, st, p->n, (float4*)s->pos, (float4*)s->vel, (float4*)s->pred,
+           (float4*)s->lambda);
+  Ws w = bind_ws(workspace_layout(*p), s->work);
+  return build_grid(*p, *s, w, false, 0, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,535 @@
+// Surface path on gfx950: density splat (shaders/surface_splat.glsl), GPU
+// marching cubes (solver/marching_cubes.py) and analytic collider
+// voxelization (collision/__init__.py geometry).
+//
+// Marching cubes runs as classify -> scan -> emit.  Every lattice edge with a
+// sign change owns one vertex, numbered by its lower sample and axis, so
+// neighbouring cubes share vertices exactly as the reference's
+// (lower-corner sample, axis) key does (solver/marching_cubes.py:260).  The
+// case table is derived at compile time from the cube's geometry with the
+// reference's construction (solver/marching_cubes.py:60-173): oriented
+// faces, per-face contour segments, loops walked in first-insertion order,
+// reversed fans.  Vertex positions are computed in float64 without
+// contraction and rounded once to float32.
+#include "fx_common.h"
+
+#pragma clang fp contract(off)
+
+namespace fx {
+
+// ---- marching-cubes tables, derived at compile time -------------------------
+struct McTables {
+  int8_t tri[256][16];  // edge triples, -1 terminated
+  uint8_t ntri[256];
+  uint16_t edge_mask[256];
+  int8_t walk[12][3];   // lower corner, upper corner, axis
+  int8_t corner[8][3];
+};
+
+constexpr int kCorners[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+constexpr int kEdges[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                               {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+constexpr int kFaces[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                              {3, 2, 6, 7}, {0, 3, 7, 4}, {1, 2, 6, 5}};
+
+constexpr int mc_edge_between(int a, int b) {
+  for (int e = 0; e < 12; ++e)
+    if ((kEdges[e][0] == a && kEdges[e][1] == b) || (kEdges[e][0] == b && kEdges[e][1] == a)) return e;
+  return -1;
+}
+
+constexpr McTables build_mc_tables() {
+  McTables t{};
+  // faces wound counter-clockwise seen from outside (Newell normal . outward)
+  int faces[6][4] = {};
+  for (int f = 0; f < 6; ++f) {
+    double nx = 0, ny = 0, nz = 0, cx = 0, cy = 0, cz = 0;
+    for (int i = 0; i < 4; ++i) {
+      const int* a = kCorners[kFaces[f][i]];
+      const int* b = kCorners[kFaces[f][(i + 1) % 4]];
+      nx += double(a[1] - b[1]) * double(a[2] + b[2]);
+      ny += double(a[2] - b[2]) * double(a[0] + b[0]);
+      nz += double(a[0] - b[0]) * double(a[1] + b[1]);
+      cx += a[0];
+      cy += a[1];
+      cz += a[2];
+    }
+    const double out = nx * (cx / 4.0 - 0.5) + ny * (cy / 4.0 - 0.5) + nz * (cz / 4.0 - 0.5);
+    for (int c = 0; c < 4; ++c) faces[f][c] = out > 0.0 ? kFaces[f][c] : kFaces[f][3 - c];
+  }
+  for (int cs = 0; cs < 256; ++cs) {
+    bool inside[8] = {};
+    for (int c = 0; c < 8; ++c) inside[c] = (cs >> c) & 1;
+    // successor map in insertion order (face order, then segment order)
+    int skey[12] = {}, sval[12] = {}, ns = 0;
+    for (int f = 0; f < 6; ++f) {
+      bool opening[4] = {};
+      int edge[4] = {}, count = 0;
+      for (int i = 0; i < 4; ++i) {
+        const int a = faces[f][i], b = faces[f][(i + 1) % 4];
+        if (inside[a] && !inside[b]) {
+          opening[count] = true;
+          edge[count++] = mc_edge_between(a, b);
+        } else if (inside[b] && !inside[a]) {
+          opening[count] = false;
+          edge[count++] = mc_edge_between(a, b);
+        }
+      }
+      for (int i = 0; i < count; ++i) {
+        if (!opening[i]) continue;
+        for (int step = 1; step < count; ++step) {
+          const int o = (i + step) % count;
+          if (!opening[o]) {
+            int found = -1;
+            for (int q = 0; q < ns; ++q)
+              if (skey[q] == edge[i]) found = q;
+            if (found >= 0) {
+              sval[found] = edge[o];
+            } else {
+              skey[ns] = edge[i];
+              sval[ns++] = edge[o];
+            }
+            break;
+          }
+        }
+      }
+    }
+    bool visited[12] = {};
+    int nt = 0;
+    for (int i = 0; i < 16; ++i) t.tri[cs][i] = -1;
+    for (int q = 0; q < ns; ++q) {
+      if (visited[skey[q]]) continue;
+      int loop[12] = {}, nl = 0, e = skey[q];
+      while (!visited[e]) {
+        visited[e] = true;
+        loop[nl++] = e;
+        int nxt = -1;
+        for (int r = 0; r < ns; ++r)
+          if (skey[r] == e) nxt = sval[r];
+        e = nxt;
+      }
+      for (int i = 1; i < nl - 1; ++i) {
+        t.tri[cs][3 * nt] = (int8_t)loop[0];
+        t.tri[cs][3 * nt + 1] = (int8_t)loop[i + 1];
+        t.tri[cs][3 * nt + 2] = (int8_t)loop[i];
+        ++nt;
+      }
+    }
+    t.ntri[cs] = (uint8_t)nt;
+    uint16_t mask = 0;
+    for (int e = 0; e < 12; ++e)
+      if (inside[kEdges[e][0]] != inside[kEdges[e][1]]) mask = (uint16_t)(mask | (1u << e));
+    t.edge_mask[cs] = mask;
+  }
+  for (int e = 0; e < 12; ++e) {
+    const int a = kEdges[e][0], b = kEdges[e][1];
+    int axis = 0;
+    while (kCorners[a][axis] == kCorners[b][axis]) ++axis;
+    const bool a_lo = kCorners[a][axis] < kCorners[b][axis];
+    t.walk[e][0] = (int8_t)(a_lo ? a : b);
+    t.walk[e][1] = (int8_t)(a_lo ? b : a);
+    t.walk[e][2] = (int8_t)axis;
+  }
+  for (int c = 0; c < 8; ++c)
+    for (int a = 0; a < 3; ++a) t.corner[c][a] = (int8_t)kCorners[c][a];
+  return t;
+}
+
+static constexpr McTables kMcHost = build_mc_tables();
+__constant__ McTables kMc = build_mc_tables();
+
+// ---- splat (shaders/surface_splat.glsl:32-97) ------------------------------
+// One thread per sample (x fastest), gathering the 5x5x5 cells around the
+// sample's clamped cell as 25 contiguous x-runs of sorted slots; pos_s holds
+// the finalized positions in the grid's sorted order.
+__global__ __launch_bounds__(256) void k_splat(FxSurface sf, Grid g, float mass, float rho0,
+                                               float h2, float coef, int max_scan, Collider col,
+                                               const float4* __restrict__ pos_s,
+                                               const int32_t* __restrict__ off,
+                                               float* __restrict__ field, long long S) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int nx = sf.dims[0], ny = sf.dims[1];
+  const int gx = (int)(s % nx), gy = (int)((s / nx) % ny), gz = (int)(s / ((long long)nx * ny));
+  const float px = sf.lo[0] + (float)gx * sf.spacing;
+  const float py = sf.lo[1] + (float)gy * sf.spacing;
+  const float pz = sf.lo[2] + (float)gz * sf.spacing;
+  int bx, by, bz;
+  cell_coord(g, px, py, pz, bx, by, bz);
+  float acc = 0.0f;
+  {
+#pragma clang fp contract(fast)
+    for_each_run<2>(g, off, max_scan, bx, by, bz, [&](int a, int e) {
+      for (int q = a; q < e; ++q) {
+        const float4 pj = pos_s[q];
+        const float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < h2) {
+          const float d = h2 - r2;
+          acc += d * d * d;
+        }
+      }
+    });
+  }
+  float value = (mass * (coef * acc)) / fmaxf(rho0, 1e-6f);
+  if (col.occ) {
+    int cx, cy, cz;
+    collider_coord(col, px, py, pz, cx, cy, cz);
+    if (collider_occupied(col, cx, cy, cz)) value = 0.0f;
+  }
+  field[s] = value;
+}
+
+__global__ void k_gather_pos(int n, const int32_t* __restrict__ sidx, const float4* __restrict__ pos,
+                             float4* __restrict__ pos_s) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  pos_s[t] = pos[sidx[t]];
+}
+
+// ---- marching cubes -------------------------------------------------------
+constexpr int kMcBlock = 256;
+
+struct McLattice {
+  int nx, ny, nz;
+  long long S;
+  float iso;
+};
+
+__device__ __forceinline__ int wave_block_scan(int v, int* lds, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  if (lane == 63) lds[wave] = v;
+  __syncthreads();
+  int add = 0;
+  for (int w = 0; w < wave; ++w) add += lds[w];
+  *total = lds[0] + lds[1] + lds[2] + lds[3];
+  __syncthreads();
+  return v + add;
+}
+
+// Per sample: the sign-change flags of its +x / +y / +z lattice edges (the
+// vertices it owns) and, if it is a cube's lower corner, the cube's case.
+__global__ __launch_bounds__(kMcBlock) void k_mc_classify(McLattice L, const float* __restrict__ f,
+                                                          uint16_t* __restrict__ code,
+                                                          int32_t* __restrict__ bv,
+                                                          int32_t* __restrict__ bt,
+                                                          uint8_t* __restrict__ cases) {
+  __shared__ int lds[4];
+  const long long q = (long long)blockIdx.x * kMcBlock + threadIdx.x;
+  int nv = 0, nt = 0;
+  if (q < L.S) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const long long nxy = (long long)nx * ny;
+    const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / nxy);
+    const bool in0 = f[q] >= L.iso;
+    int flags = 0;
+    if (i + 1 < nx && ((f[q + 1] >= L.iso) != in0)) flags |= 1;
+    if (j + 1 < ny && ((f[q + nx] >= L.iso) != in0)) flags |= 2;
+    if (k + 1 < nz && ((f[q + nxy] >= L.iso) != in0)) flags |= 4;
+    int cs = 0;
+    if (i + 1 < nx && j + 1 < ny && k + 1 < nz) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const long long o = kMc.corner[c][0] + (long long)kMc.corner[c][1] * nx + kMc.corner[c][2] * nxy;
+        if (f[q + o] >= L.iso) cs |= 1 << c;
+      }
+      nt = kMc.ntri[cs];
+      if (cases) cases[((long long)k * (ny - 1) + j) * (nx - 1) + i] = (uint8_t)cs;
+    }
+    nv = __popc(flags);
+    code[q] = (uint16_t)(flags | (cs << 3));
+  }
+  int tv, tt;
+  wave_block_scan(nv, lds, &tv);
+  wave_block_scan(nt, lds, &tt);
+  if (threadIdx.x == 0) {
+    bv[blockIdx.x] = tv;
+    bt[blockIdx.x] = tt;
+  }
+}
+
+// Per sample: global vertex / triangle bases, and its vertices (float64
+// interpolation of solver/marching_cubes.py:263-275, rounded to float32).
+__global__ __launch_bounds__(kMcBlock) void k_mc_vertices(McLattice L, const float* __restrict__ f,
+                                                          const uint16_t* __restrict__ code,
+                                                          const int32_t* __restrict__ bv_off,
+                                                          const int32_t* __restrict__ bt_off,
+                                                          uint32_t* __restrict__ vbase,
+                                                          uint32_t* __restrict__ tbase,
+                                                          double ox, double oy, double oz,
+                                                          double spacing,
+                                                          float* __restrict__ verts,
+                                                          long long vcap) {
+  __shared__ int lds[4];
+  const long long q = (long long)blockIdx.x * kMcBlock + threadIdx.x;
+  int flags = 0, cs = 0;
+  if (q < L.S) {
+    const int c = code[q];
+    flags = c & 7;
+    cs = c >> 3;
+  }
+  const int nv = __popc(flags), nt = kMc.ntri[cs];
+  int tv, tt;
+  const int iv = wave_block_scan(nv, lds, &tv) - nv + bv_off[blockIdx.x];
+  const int it = wave_block_scan(nt, lds, &tt) - nt + bt_off[blockIdx.x];
+  if (q >= L.S) return;
+  vbase[q] = (uint32_t)iv;
+  tbase[q] = (uint32_t)it;
+  if (!flags) return;
+  const int nx = L.nx, ny = L.ny;
+  const long long nxy = (long long)nx * ny;
+  const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / nxy);
+  const double lo_v = (double)f[q];
+  const double iso = (double)L.iso;
+  int r = 0;
+  for (int axis = 0; axis < 3; ++axis) {
+    if (!(flags & (1 << axis))) continue;
+    const long long stride = axis == 0 ? 1 : (axis == 1 ? nx : nxy);
+    const double hi_v = (double)f[q + stride];
+    const double delta = hi_v - lo_v;
+    const double t = delta == 0.0 ? 0.5 : (iso - lo_v) / delta;
+    double pt[3] = {ox + (double)i * spacing, oy + (double)j * spacing, oz + (double)k * spacing};
+    pt[axis] += t * spacing;
+    const long long v = (long long)iv + r;
+    if (v < vcap) {
+      verts[3 * v] = (float)pt[0];
+      verts[3 * v + 1] = (float)pt[1];
+      verts[3 * v + 2] = (float)pt[2];
+    }
+    ++r;
+  }
+}
+
+// Per cube: its triangles from the case table, vertex indices looked up by
+// (lower-corner sample, axis).
+__global__ __launch_bounds__(kMcBlock) void k_mc_triangles(McLattice L,
+                                                           const uint16_t* __restrict__ code,
+                                                           const uint32_t* __restrict__ vbase,
+                                                           const uint32_t* __restrict__ tbase,
+                                                           int32_t* __restrict__ tris, long long tcap) {
+  const long long q = (long long)blockIdx.x * kMcBlock + threadIdx.x;
+  if (q >= L.S) return;
+  const int cs = code[q] >> 3;
+  const int nt = kMc.ntri[cs];
+  if (!nt) return;
+  const long long nx = L.nx, nxy = (long long)L.nx * L.ny;
+  const long long base = tbase[q];
+  for (int m = 0; m < nt; ++m) {
+    int idx[3];
+    for (int e3 = 0; e3 < 3; ++e3) {
+      const int e = kMc.tri[cs][3 * m + e3];
+      const int lc = kMc.walk[e][0], axis = kMc.walk[e][2];
+      const long long q2 = q + kMc.corner[lc][0] + kMc.corner[lc][1] * nx + kMc.corner[lc][2] * nxy;
+      const int fl = code[q2] & 7;
+      idx[e3] = (int)vbase[q2] + __popc(fl & ((1 << axis) - 1));
+    }
+    const long long tt = base + m;
+    if (tt < tcap) {
+      tris[3 * tt] = idx[0];
+      tris[3 * tt + 1] = idx[1];
+      tris[3 * tt + 2] = idx[2];
+    }
+  }
+}
+
+__global__ void k_mc_counts(const int32_t* bv_off, const int32_t* bt_off, int nb, int64_t* counts) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    counts[0] = bv_off[nb];
+    counts[1] = bt_off[nb];
+  }
+}
+
+struct McWs {
+  size_t code, vbase, tbase, bv, bt, bv_off, bt_off, part, total;
+};
+
+static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static McWs mc_layout(long long S) {
+  McWs w;
+  const long long nb = (S + kMcBlock - 1) / kMcBlock + 1;
+  size_t o = 0;
+  w.code = o;   o += al((size_t)S * 2 + 2);
+  w.vbase = o;  o += al((size_t)S * 4 + 4);
+  w.tbase = o;  o += al((size_t)S * 4 + 4);
+  w.bv = o;     o += al((size_t)nb * 4);
+  w.bt = o;     o += al((size_t)nb * 4);
+  w.bv_off = o; o += al((size_t)(nb + 1) * 4);
+  w.bt_off = o; o += al((size_t)(nb + 1) * 4);
+  w.part = o;   o += al(scan_partials(nb) * 4 + 4);
+  w.total = o;
+  return w;
+}
+
+// ---- voxelizers --------------------------------------------------------------
+// Occupancy of voxel centres origin + (i + 0.5) * voxel (float64 as the
+// reference computes them, collision/__init__.py:353-366).
+struct VoxShape {
+  int kind;  // 0 box, 1 sphere, 2 vertical cylinder
+  double a[3], b[3];
+  double r, height;
+};
+
+__global__ void k_voxelize(int nx, int ny, int nz, double ox, double oy, double oz, double voxel,
+                           VoxShape sh, uint8_t* __restrict__ grid, int accumulate) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)nx * ny * nz;
+  if (q >= total) return;
+  const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((long long)nx * ny));
+  const double x = ox + (i + 0.5) * voxel, y = oy + (j + 0.5) * voxel, z = oz + (k + 0.5) * voxel;
+  bool in = false;
+  if (sh.kind == 0) {
+    in = x > sh.a[0] && x < sh.b[0] && y > sh.a[1] && y < sh.b[1] && z > sh.a[2] && z < sh.b[2];
+  } else if (sh.kind == 1) {
+    const double dx = x - sh.a[0], dy = y - sh.a[1], dz = z - sh.a[2];
+    in = dx * dx + dy * dy + dz * dz < sh.r * sh.r;
+  } else {
+    const double dx = x - sh.a[0], dy = y - sh.a[1];
+    in = dx * dx + dy * dy < sh.r * sh.r && z > sh.a[2] && z < sh.a[2] + sh.height;
+  }
+  if (accumulate) {
+    if (in) grid[q] = 1;
+  } else {
+    grid[q] = in ? 1 : 0;
+  }
+}
+
+static int voxelize(const int32_t dims[3], const float origin[3], float voxel, const VoxShape& sh,
+                    uint8_t* grid, int accumulate, hipStream_t st) {
+  if (!dims || !origin || !grid) return fail(FX_EINVAL, "voxelize: null argument");
+  if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return fail(FX_EINVAL, "voxelize: dims must be >= 1");
+  if (!(voxel > 0.0f)) return fail(FX_EINVAL, "voxelize: voxel size must be > 0");
+  const long long total = (long long)dims[0] * dims[1] * dims[2];
+  hipLaunchKernelGGL(k_voxelize, dim3(grid_for(total)), dim3(kBlock), 0, st, dims[0], dims[1], dims[2],
+                     (double)origin[0], (double)origin[1], (double)origin[2], (double)voxel, sh, grid,
+                     accumulate);
+  return check_launch("voxelize");
+}
+
+}  // namespace fx
+
+using namespace fx;
+
+extern "C" {
+
+int fx_splat(const FxParams* p, const FxSurface* surf, const FxState* s, float* field, void* stream) {
+  if (!p || !surf || !s || !field) return fail(FX_EINVAL, "fx_splat: null argument");
+  if (surf->dims[0] < 1 || surf->dims[1] < 1 || surf->dims[2] < 1)
+    return fail(FX_EINVAL, "fx_splat: surface dims must be >= 1");
+  if (!s->work || s->work_bytes < workspace_layout(*p).total)
+    return fail(FX_ENOMEM, "fx_splat: workspace missing or too small");
+  if (!grid_ready(s->work, p->n, p->n_cells))
+    return fail(FX_EINVAL, "fx_splat gathers through the last grid: build one first");
+  if (p->n > 0 && !s->pos) return fail(FX_EINVAL, "fx_splat: pos is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  const Workspace wl = workspace_layout(*p);
+  char* b = (char*)s->work;
+  const int32_t* sidx = (const int32_t*)(b + wl.sidx);
+  const int32_t* off = (const int32_t*)(b + wl.off);
+  float4* pos_s = (float4*)(b + wl.pos_s);
+  if (p->n > 0)
+    hipLaunchKernelGGL(k_gather_pos, dim3(grid_for(p->n)), dim3(kBlock), 0, st, p->n, sidx,
+                       (const float4*)s->pos, pos_s);
+  const long long S = (long long)surf->dims[0] * surf->dims[1] * surf->dims[2];
+  const float kr = surf->kernel_radius;
+  const float coef = poly6_coef(kr);
+  Collider col = make_collider(*p, s->collider);
+  hipLaunchKernelGGL(k_splat, dim3(grid_for(S)), dim3(256), 0, st, *surf, make_grid(*p), p->mass,
+                     p->rest_density, kr * kr, coef, p->max_cell_scan > 0 ? p->max_cell_scan : 512, col,
+                     (const float4*)pos_s, off, field, S);
+  return check_launch("fx_splat");
+}
+
+size_t fx_mc_workspace_bytes(const int32_t dims[3]) {
+  if (!dims || dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return 0;
+  return mc_layout((long long)dims[0] * dims[1] * dims[2]).total;
+}
+
+int fx_mc_extract(const float* field, const int32_t dims[3], const double origin[3], double spacing,
+                  float iso, float* verts, int64_t vcap, int32_t* tris, int64_t tcap, int64_t* counts,
+                  uint8_t* cases, void* work, size_t work_bytes, void* stream) {
+  if (!field || !dims || !origin || !counts || !work)
+    return fail(FX_EINVAL, "fx_mc_extract: null argument");
+  if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return fail(FX_EINVAL, "fx_mc_extract: dims must be >= 1");
+  if (vcap < 0 || tcap < 0) return fail(FX_EINVAL, "fx_mc_extract: negative capacity");
+  if ((vcap > 0 && !verts) || (tcap > 0 && !tris)) return fail(FX_EINVAL, "fx_mc_extract: null output");
+  const long long S = (long long)dims[0] * dims[1] * dims[2];
+  if (S > (1LL << 28)) return fail(FX_ERANGE, "fx_mc_extract: %lld samples exceed 2^28", S);
+  const McWs wl = mc_layout(S);
+  if (work_bytes < wl.total) return fail(FX_ENOMEM, "fx_mc_extract: workspace %zu < %zu", work_bytes, wl.total);
+  hipStream_t st = (hipStream_t)stream;
+  char* b = (char*)work;
+  uint16_t* code = (uint16_t*)(b + wl.code);
+  uint32_t* vbase = (uint32_t*)(b + wl.vbase);
+  uint32_t* tbase = (uint32_t*)(b + wl.tbase);
+  int32_t* bv = (int32_t*)(b + wl.bv);
+  int32_t* bt = (int32_t*)(b + wl.bt);
+  int32_t* bv_off = (int32_t*)(b + wl.bv_off);
+  int32_t* bt_off = (int32_t*)(b + wl.bt_off);
+  int32_t* part = (int32_t*)(b + wl.part);
+  const long long nb = (S + kMcBlock - 1) / kMcBlock;
+  McLattice L{dims[0], dims[1], dims[2], S, iso};
+  hipLaunchKernelGGL(k_mc_classify, dim3((unsigned)nb), dim3(kMcBlock), 0, st, L, field, code, bv, bt, cases);
+  FX_TRY(exclusive_scan_i32(bv, bv_off, nb, part, st));
+  FX_TRY(exclusive_scan_i32(bt, bt_off, nb, part, st));
+  hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)nb), dim3(kMcBlock), 0, st, L, field,
+                     (const uint16_t*)code, (const int32_t*)bv_off, (const int32_t*)bt_off, vbase, tbase,
+                     origin[0], origin[1], origin[2], spacing, verts, (long long)vcap);
+  hipLaunchKernelGGL(k_mc_triangles, dim3((unsigned)nb), dim3(kMcBlock), 0, st, L, (const uint16_t*)code,
+                     (const uint32_t*)vbase, (const uint32_t*)tbase, tris, (long long)tcap);
+  hipLaunchKernelGGL(k_mc_counts, dim3(1), dim3(64), 0, st, (const int32_t*)bv_off, (const int32_t*)bt_off,
+                     (int)nb, counts);
+  return check_launch("fx_mc_extract");
+}
+
+// Host copy of the derived case table (edge triples, -1 terminated) and the
+// per-case sign-change edge mask, for checking the derivation.
+int fx_mc_tables(int8_t* tri_out, uint16_t* edge_mask_out) {
+  if (!tri_out || !edge_mask_out) return fail(FX_EINVAL, "fx_mc_tables: null argument");
+  for (int c = 0; c < 256; ++c) {
+    for (int i = 0; i < 16; ++i) tri_out[c * 16 + i] = kMcHost.tri[c][i];
+    edge_mask_out[c] = kMcHost.edge_mask[c];
+  }
+  return FX_OK;
+}
+
+int fx_voxelize_boxes(const int32_t dims[3], const float origin[3], float voxel, const float* boxes,
+                      int k, uint8_t* grid, int accumulate, void* stream) {
+  if (k < 0 || (k > 0 && !boxes)) return fail(FX_EINVAL, "fx_voxelize_boxes: bad box list");
+  hipStream_t st = (hipStream_t)stream;
+  if (k == 0 && !accumulate) {
+    VoxShape none{0, {1, 1, 1}, {0, 0, 0}, 0, 0};  // empty box: clears the grid
+    return voxelize(dims, origin, voxel, none, grid, 0, st);
+  }
+  for (int b = 0; b < k; ++b) {
+    VoxShape sh{0,
+                {boxes[6 * b], boxes[6 * b + 1], boxes[6 * b + 2]},
+                {boxes[6 * b + 3], boxes[6 * b + 4], boxes[6 * b + 5]},
+                0.0,
+                0.0};
+    FX_TRY(voxelize(dims, origin, voxel, sh, grid, (b > 0 || accumulate) ? 1 : 0, st));
+  }
+  return FX_OK;
+}
+
+int fx_voxelize_sphere(const int32_t dims[3], const float origin[3], float voxel, const float center[3],
+                       float radius, uint8_t* grid, int accumulate, void* stream) {
+  if (!center) return fail(FX_EINVAL, "fx_voxelize_sphere: null centre");
+  VoxShape sh{1, {center[0], center[1], center[2]}, {0, 0, 0}, radius, 0.0};
+  return voxelize(dims, origin, voxel, sh, grid, accumulate, (hipStream_t)stream);
+}
+
+int fx_voxelize_cylinder(const int32_t dims[3], const float origin[3], float voxel, const float base[3],
+                         float radius, float height, uint8_t* grid, int accumulate, void* stream) {
+  if (!base) return fail(FX_EINVAL, "fx_voxelize_cylinder: null base");
+  VoxShape sh{2, {base[0], base[1], base[2]}, {0, 0, 0}, radius, height};
+  return voxelize(dims, origin, voxel, sh, grid, accumulate, (hipStream_t)stream);
+}
+
+}  // extern "C"
