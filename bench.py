@@ -1,0 +1,211 @@
+"""Benchmark: particle-substeps/s of the PBF substep on one MI355X.
+
+Workload (default): the 1.64M-particle scene - 2 m domain, h = 0.025 m,
+160 x 160 x 64 jittered lattice (40% fill), 4 Jacobi iterations, cohesion on,
+XSPH viscosity 0.01, a 0.3 m pillar and a 1.0 x 0.1 x 0.5 m half-wall on an
+80^3 collider grid, dt from 24 fps (4 substeps per frame).  One step = one
+full substep (normal, predict, grid build, 4 x (lambda, delta+apply),
+velocity, XSPH, finalize) with every input resident in HBM.  The surface
+(256-resolution splat + marching cubes) is timed separately.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+
+Prints one JSON line.  The roofline entry is for the kernel with the largest
+share of substep time, timed live with HIP events (fx_profile) over a second
+timed region of the same substeps.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(kernel, n, c):
+    """Compulsory HBM bytes of one launch: each array element read once and
+    written once, float32 SoA accounting (vec3 = 12 B, scalar = 4 B, int = 4 B),
+    neighbour gathers counted once (they re-read the same arrays)."""
+    per_particle = {
+        "k_lambda": 24 + 8.0 * c / n,       # pred(12) + keys/idx(8) -> rho, lambda(8); cell ranges 8C
+        "k_delta": 32 + 8.0 * c / n + 12,   # pred, lambda -> delta, plus the fused apply write (12)
+        "k_xsph<true>": 44 + 8.0 * c / n + 24,  # pred, vel, rho -> dv; fused finalize writes pos, vel
+        "k_normal": 36 + 8.0 * c / n,
+        "k_predict_key": 48 + 8,            # pos, vel -> vel, pred; key + rank
+    }.get(kernel)
+    if per_particle is None:
+        return None, None
+    return per_particle * n, per_particle
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (profiles/<kernel>_traffic.txt: 'fetch_bytes=' and 'write_bytes=' lines),
+    or None."""
+    f = ROOT / "profiles" / f"{kernel.replace('<', '_').replace('>', '')}_traffic.txt"
+    if not f.exists():
+        return None
+    vals = {}
+    for line in f.read_text().splitlines():
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except ValueError:
+                pass
+    if "fetch_bytes" in vals and "write_bytes" in vals:
+        return vals["fetch_bytes"] + vals["write_bytes"]
+    return None
+
+
+def cpu_baseline(cfg, dt, substeps):
+    """The oracle (float32 C restatement of the reference passes, bitonic sort,
+    OpenMP over the host cores) on `substeps` substeps of the same scene."""
+    import oracle
+
+    from flowx_amd.collision import grid_geometry, occupancy_cpu
+    from flowx_amd.config import seed_positions
+
+    pos0 = seed_positions(cfg)
+    origin, voxel, dims = grid_geometry(cfg.lo, cfg.hi, cfg.extra["collider_resolution"])
+    col = occupancy_cpu(origin, voxel, dims, cfg.extra["colliders"]).reshape(-1)
+    p = cfg.params(dt)
+    p.collider_dims[:] = list(dims)
+    p.collider_voxel = voxel
+    st = oracle.State(p, pos0, precision="f32", sort_mode="bitonic")
+    st.substep(col)  # warm-up (also builds the previous grid the normal pass reads)
+    t0 = time.perf_counter()
+    for _ in range(substeps):
+        st.substep(col)
+    el = time.perf_counter() - t0
+    cores = oracle.num_threads()
+    return {
+        "value": cfg.particle_count * substeps / el,
+        "unit": "particle-substeps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{substeps} substeps of the same {cfg.particle_count}-particle scene, oracle "
+                  f"(CPU restatement of the reference passes incl. the bitonic sort), float32, "
+                  f"{cores} OpenMP threads, {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-substeps", type=int, default=2)
+    ap.add_argument("--literal-update", action="store_true",
+                    help="the reference's literal position correction (diverges at these masses)")
+    ap.add_argument("--no-surface", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    import __graft_entry__
+
+    fx = __graft_entry__.load_package()
+    from flowx_amd import _lib
+
+    cfg = fx.scale_config(literal_update=args.literal_update)
+    solver = fx.Solver(cfg, keep_scratch=False)
+    solver.reset()
+    _, dt = cfg.substep_dt(1.0 / cfg.extra["fps"])
+    n, c = cfg.particle_count, cfg.cell_count
+    print(f"[bench] {n} particles, {c} cells, dt {dt:.6f}, warmup {args.warmup}, steps {args.steps}",
+          file=sys.stderr, flush=True)
+
+    solver.substeps_n(dt, args.warmup)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    solver.substeps_n(dt, args.steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_time(e1)
+    ms_per_step = gpu_ms / args.steps
+    value = n * args.steps / (gpu_ms / 1e3)
+    print(f"[bench] {ms_per_step:.3f} ms/substep (wall {wall * 1e3 / args.steps:.3f})", file=sys.stderr, flush=True)
+
+    # per-kernel durations over a second timed region (same substeps)
+    prof_steps = max(4, min(args.steps, 20))
+    _lib.profile(True)
+    solver.substeps_n(dt, prof_steps)
+    torch.cuda.synchronize()
+    rep = _lib.profile_report()
+    _lib.profile(False)
+    kernels = {k: {"launches": cnt, "avg_us": 1e3 * ms / cnt, "ms_per_substep": ms / prof_steps}
+               for k, (cnt, ms) in rep.items()}
+    dominant = max(kernels, key=lambda k: kernels[k]["ms_per_substep"])
+    alg, per_particle = algorithmic_bytes(dominant, n, c)
+    avg_s = kernels[dominant]["avg_us"] * 1e-6
+    roofline = None
+    if alg is not None:
+        achieved = alg / avg_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(dominant),
+                    "kernel": dominant, "avg_us": round(kernels[dominant]["avg_us"], 2),
+                    "bytes_per_particle": round(per_particle, 2)}
+
+    surface = None
+    if not args.no_surface:
+        surf = fx.Surface(solver, cfg.extra["surface_resolution"], 0.5, max_samples=1 << 30)
+        surf.update()
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            verts, tris = surf.update()
+        torch.cuda.synchronize()
+        surface = {"dims": list(surf.config.dims), "ms": round((time.perf_counter() - s0) * 1e3 / reps, 3),
+                   "vertices": int(verts.shape[0]), "triangles": int(tris.shape[0])}
+
+    rho = None
+    base = None
+    if not args.no_cpu_baseline:
+        base = cpu_baseline(cfg, dt, args.cpu_substeps)
+
+    out = {
+        "metric": "particle-substeps/s",
+        "value": round(value, 1),
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic jittered lattice (seeded MT19937), analytic colliders",
+        "config": {
+            "workload": f"PBF substep, {n} particles (160x160x64, h=0.025 m, 2 m domain), 4 Jacobi iterations, "
+                        f"cohesion on, XSPH 0.01, pillar + half-wall on 80^3 voxels, "
+                        f"update: {cfg.extra.get('update', 'reference literal')}",
+            "particles": n, "cells": c, "dt": dt,
+        },
+        "roofline": roofline,
+        "cpu_baseline": base,
+        "kernels": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kernels.items()},
+        "surface": surface,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

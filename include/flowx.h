@@ -73,7 +73,17 @@ typedef struct FxParams {
   float collider_voxel;      /* i_collider.w  0 = no collider                     */
   int32_t iterations;        /* Jacobi iterations per substep (solver/sph.py:510) */
   int32_t max_cell_scan;     /* MAX_CELL_SCAN (shaders/sph_common.glsl:27-30)     */
+  int32_t flags;             /* FX_FLAG_* (no reference counterpart; 0 = the      */
+                             /* reference's exact pass semantics)                 */
 } FxParams;
+
+/* Position correction weighted by the particle mass,
+ * dp_i = (m / rho0) sum_j (lambda_i + lambda_j + s_corr) gradW_ij.  The
+ * reference's shaders/sph_delta.glsl:81-84 omits m while its lambda
+ * (shaders/sph_lambda.glsl:58-74) includes it, so its Jacobi step is 1/m times
+ * the Newton step and diverges whenever m < 1 (every default scene).  Off by
+ * default: set it only for runs that need the constraint to converge. */
+#define FX_FLAG_MASS_WEIGHTED_DELTA 1
 
 /* Surface lattice of the density splat (solver/surface.py:97-140 resolve(),
  * repurposed slots of shaders/surface_splat.glsl:13-27). */
@@ -130,6 +140,26 @@ enum FxPass {
 /* Replaces the error reporting of solver/gpu_util.py:92-103 / sph.py:926-933. */
 const char* fx_last_error(void);
 int fx_abi_version(void);
+
+/* Per-kernel timing with HIP events (no reference counterpart; the reference
+ * times only whole frames, solver/sph.py:517-545).  fx_profile(1) starts
+ * bracketing every kernel launch with an event pair on its stream;
+ * fx_profile_report() waits for the recorded events and writes one line
+ * "<kernel> <launches> <total_ms>" per kernel into buf, then clears the
+ * record.  Leave it off under HIP graph capture. */
+int fx_profile(int enable);
+
+/* Launch tuning of the neighbour passes (no reference counterpart), set by
+ * the autotuner.  list_skin: neighbour-list margin in units of h (0 = always
+ * scan the full 27-cell stencil); block: 64, 128 or 256 threads.  Results do
+ * not depend on either. */
+typedef struct FxTuning {
+  float list_skin;
+  int32_t block;
+} FxTuning;
+int fx_set_tuning(const FxTuning* t);
+int fx_get_tuning(FxTuning* t);
+int fx_profile_report(char* buf, size_t len);
 
 /* Bytes of `work` the solver needs for these params (internal sorted-order
  * buffers, counters, scan partials).  Replaces the scratch allocations of

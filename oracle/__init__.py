@@ -58,6 +58,7 @@ class FxParams(ctypes.Structure):
         ("collider_voxel", ctypes.c_float),
         ("iterations", ctypes.c_int32),
         ("max_cell_scan", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
     ]
 
 

@@ -1,0 +1,73 @@
+"""CPU-side checks of the C ABI: the library loads, exports every entry point
+include/flowx.h declares, its ctypes structs match the header's layout, and
+argument validation errors come back as codes (no GPU calls are made)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared():
+    text = (ROOT / "include" / "flowx.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(fx):
+    lib = fx._lib.lib()
+    missing = [name for name in _declared() if not hasattr(lib, name)]
+    assert not missing, f"declared in flowx.h but not exported: {missing}"
+    assert lib.fx_abi_version() == 1
+
+
+def test_every_declared_symbol_has_a_ctypes_signature(fx):
+    missing = [name for name in _declared() if name not in fx._lib.SIGNATURES]
+    assert not missing
+
+
+def test_struct_layouts_match_between_product_and_oracle(fx, oracle):
+    for a, b in ((fx._lib.FxParams, oracle.FxParams), (fx._lib.FxSurface, oracle.FxSurface)):
+        assert ctypes.sizeof(a) == ctypes.sizeof(b)
+        assert [f[0] for f in a._fields_] == [f[0] for f in b._fields_]
+    assert ctypes.sizeof(fx._lib.FxParams) == 30 * 4  # thirty 4-byte lanes
+
+
+def test_workspace_size_grows_with_particles(fx):
+    cfg = fx.dam_break_config()
+    p = cfg.params(0.01)
+    a = fx._lib.lib().fx_workspace_bytes(ctypes.byref(p))
+    p.n = p.n * 2
+    b = fx._lib.lib().fx_workspace_bytes(ctypes.byref(p))
+    assert b > a > 16 * cfg.particle_count
+
+
+def test_invalid_arguments_return_codes_without_gpu(fx):
+    L = fx._lib
+    cfg = fx.dam_break_config()
+    p = cfg.params(0.01)
+    s = L.FxState()
+    # no workspace -> FX_EINVAL before any device work
+    rc = L.lib().fx_substep(ctypes.byref(p), ctypes.byref(s), None)
+    assert rc == -1
+    assert b"workspace" in L.lib().fx_last_error()
+    p.cells[0] = 0
+    rc = L.lib().fx_build_grid(ctypes.byref(p), ctypes.byref(s), None)
+    assert rc == -1 and b"cell dims" in L.lib().fx_last_error()
+    with pytest.raises(RuntimeError):
+        L.check(rc, "fx_build_grid")
+
+
+def test_mc_table_of_library_matches_reference_golden(fx):
+    import numpy as np
+
+    g = np.load(ROOT / "tests" / "golden" / "mc_golden.npz")
+    tri = np.zeros((256, 16), np.int8)
+    mask = np.zeros(256, np.uint16)
+    fx._lib.check(fx._lib.lib().fx_mc_tables(ctypes.c_void_p(tri.ctypes.data), ctypes.c_void_p(mask.ctypes.data)),
+                  "fx_mc_tables")
+    assert np.array_equal(tri, g["tri_table"])
+    assert np.array_equal(mask, g["edge_table"])

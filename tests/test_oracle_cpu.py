@@ -1,0 +1,272 @@
+"""Pin the oracle (CPU restatement) before trusting it as the parity reference.
+
+Each check is chosen so a plausible transliteration mistake (a dropped term,
+a wrong sign or index, a swapped operand) fails it: CPython's own RNG, the
+bitonic network's hand-derived answer, the reference's marching-cubes golden
+vectors, closed forms, invariants and brute force.  No GPU is used.
+"""
+
+import math
+import random
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+GOLDEN = np.load(ROOT / "tests" / "golden" / "mc_golden.npz")
+
+
+# ---- RNG and seeding (solver/sph.py:316-347) ---------------------------------
+
+def test_mt19937_matches_cpython_random(oracle):
+    r = random.Random(0)
+    expect = np.array([r.random() for _ in range(5000)])
+    assert np.array_equal(oracle.mt_random(0, 5000), expect)
+    assert oracle.mt_random(0, 3).tolist() == [0.8444218515250481, 0.7579544029403025, 0.420571580830845]
+
+
+def test_seed_lattice_matches_reference_loop(fx, oracle):
+    """The seeder's k -> j -> i loop with uniform() draws x, y, z, written out
+    with CPython's random module exactly as solver/sph.py:329-346 does."""
+    cfg = fx.lattice_config((-1.0, -1.0, 0.0), (1.0, 1.0, 2.0), 0.1, (5, 4, 3))
+    from flowx_amd.config import lattice_base
+
+    base = lattice_base(cfg.lo, cfg.spacing)
+    rng = random.Random(0)
+    jitter = cfg.spacing * 0.1
+    expect = []
+    for k in range(3):
+        for j in range(4):
+            for i in range(5):
+                expect.append((base[0] + i * cfg.spacing + rng.uniform(-jitter, jitter),
+                               base[1] + j * cfg.spacing + rng.uniform(-jitter, jitter),
+                               base[2] + k * cfg.spacing + rng.uniform(-jitter, jitter), 1.0))
+    expect = np.asarray(expect, np.float32)
+    assert np.array_equal(oracle.seed_lattice(base, cfg.spacing, jitter, (5, 4, 3)), expect)
+    assert np.array_equal(fx.seed_positions(cfg), expect)  # the product's numpy seeder
+
+
+# ---- bitonic network, stable sort, cell ranges ---------------------------------
+
+def test_bitonic_known_answer_equal_keys(oracle):
+    _, idx = oracle.bitonic(np.zeros(4, np.int32), np.arange(4, dtype=np.int32))
+    assert idx.tolist() == [0, 1, 3, 2]
+
+
+@pytest.mark.parametrize("P", [2, 8, 64, 1024])
+def test_bitonic_sorts_and_keeps_multisets(oracle, P):
+    rng = np.random.default_rng(P)
+    keys = rng.integers(0, 17, P).astype(np.int32)
+    k, i = oracle.bitonic(keys, np.arange(P, dtype=np.int32))
+    assert np.all(np.diff(k) >= 0)
+    assert np.array_equal(keys[i], k)
+    assert sorted(i.tolist()) == list(range(P))
+
+
+def test_stable_sort_orders_equal_keys_by_index(oracle):
+    keys = np.array([3, 1, 3, 0, 1, 3], np.int32)
+    k, i = oracle.stable_sort(keys, np.arange(6, dtype=np.int32), 3)
+    assert k.tolist() == [0, 1, 1, 3, 3, 3]
+    assert i.tolist() == [3, 1, 4, 0, 2, 5]
+
+
+def test_cell_range_brute_force(oracle):
+    rng = np.random.default_rng(5)
+    nc = 50
+    keys = np.sort(np.concatenate([rng.integers(0, nc, 300), np.full(212, nc)])).astype(np.int32)
+    start, end = oracle.cell_range(nc, keys)
+    for c in range(nc):
+        where = np.nonzero(keys == c)[0]
+        if len(where) == 0:
+            assert start[c] == -1 and end[c] == -1
+        else:
+            assert start[c] == where[0] and end[c] == where[-1] + 1
+
+
+# ---- closed forms and invariants of the passes -------------------------------
+
+def _single(fx, n_lattice=(1, 1, 1), h=0.1):
+    cfg = fx.lattice_config((-1.0, -1.0, 0.0), (1.0, 1.0, 2.0), h, n_lattice)
+    return cfg
+
+
+def test_single_particle_density_is_self_term(fx, oracle):
+    cfg = _single(fx)
+    p = cfg.params(0.01)
+    pos = np.array([[0.03, -0.02, 1.0, 1.0]], np.float32)
+    keys, idx, start, end = oracle.build_grid(p, pos, "stable")
+    for prec in ("f32", "f64"):
+        lam = oracle.lambda_(p, pos, idx, start, end, precision=prec)
+        h = np.float64(np.float32(cfg.smoothing_radius))
+        expect = float(np.float32(cfg.mass)) * 315.0 / (64.0 * math.pi * h**9) * h**6
+        assert lam[0, 0] == pytest.approx(expect, rel=1e-6)
+        # C = rho/rho0 - 1, no neighbours: denominator = eps, lambda = -C/eps
+        assert lam[0, 1] == pytest.approx(-(expect / cfg.rest_density - 1.0) / cfg.relaxation, rel=1e-6)
+
+
+def test_free_fall_predict(fx, oracle):
+    cfg = _single(fx)
+    dt = 0.01
+    p = cfg.params(dt)
+    pos = np.array([[0.0, 0.0, 1.0, 1.0]], np.float32)
+    vel, pred = oracle.predict(p, pos, np.zeros_like(pos), np.zeros_like(pos), precision="f64")
+    assert vel[0, 2] == pytest.approx(-9.81 * dt, rel=1e-6)
+    assert pred[0, 2] == pytest.approx(1.0 - 9.81 * dt * dt, rel=1e-7)
+    # speed clamp at 0.4 h / dt
+    vel2, _ = oracle.predict(p, pos, np.array([[100.0, 0, 0, 0]], np.float32), np.zeros_like(pos), "f64")
+    assert math.hypot(*vel2[0, :3]) == pytest.approx(0.4 * np.float32(cfg.smoothing_radius) / dt, rel=1e-6)
+
+
+def _jittered_block(fx, counts=(10, 10, 8)):
+    cfg = fx.lattice_config((-1.0, -1.0, 0.0), (1.0, 1.0, 2.0), 0.1, counts)
+    pos = fx.seed_positions(cfg)
+    pos[:, :3] += np.float32(0.3)
+    return cfg, pos
+
+
+def _brute(params, pos, fn):
+    """O(N^2) neighbour lists in index order (grid-free)."""
+    n = len(pos)
+    d = pos[:, None, :3].astype(np.float64) - pos[None, :, :3]
+    r2 = (d * d).sum(-1)
+    return [np.nonzero(r2[i] < float(np.float32(params.h)) ** 2)[0] for i in range(n)], d, r2
+
+
+def test_grid_lambda_equals_brute_force(fx, oracle):
+    """The grid path (27 cells, MAX_CELL_SCAN) finds exactly the pairs within h:
+    its density equals a brute-force O(N^2) sum."""
+    cfg, pos = _jittered_block(fx)
+    p = cfg.params(0.01)
+    keys, idx, start, end = oracle.build_grid(p, pos, "stable")
+    lam = oracle.lambda_(p, pos, idx, start, end, precision="f64")
+    nb, d, r2 = _brute(p, pos, None)
+    h2 = float(np.float32(p.h)) ** 2
+    coef = 315.0 / (64.0 * math.pi * float(np.float32(p.h)) ** 9)
+    m = float(np.float32(p.mass))
+    for i in range(0, len(pos), 37):
+        rho = sum(m * coef * (h2 - r2[i, j]) ** 3 for j in nb[i])
+        assert lam[i, 0] == pytest.approx(rho, rel=1e-9)
+
+
+def test_delta_sums_to_zero_for_mutual_pairs(fx, oracle):
+    """Pair antisymmetry: with s_corr the sum of all corrections vanishes
+    (every pair within h is mutual, no cell exceeds MAX_CELL_SCAN)."""
+    cfg, pos = _jittered_block(fx)
+    p = cfg.params(0.01)
+    keys, idx, start, end = oracle.build_grid(p, pos, "stable")
+    lam = oracle.lambda_(p, pos, idx, start, end, precision="f64")
+    dl = oracle.delta(p, pos, lam, idx, start, end, precision="f64")
+    total = np.abs(dl[:, :3].sum(0)).max()
+    assert total < 1e-9 * np.abs(dl[:, :3]).sum()
+    # a sign error in the pair term would break the density-increasing direction:
+    # an under-dense particle (lambda > 0) is pulled towards its neighbours
+    i = int(np.argmax(lam[:, 1]))
+    com = pos[nb := (np.linalg.norm(pos[:, :3] - pos[i, :3], axis=1) < float(p.h)), :3].mean(0) - pos[i, :3]
+    assert np.dot(dl[i, :3], com) > 0
+
+
+def test_xsph_of_uniform_velocity_is_zero(fx, oracle):
+    cfg, pos = _jittered_block(fx)
+    p = cfg.params(0.01)
+    keys, idx, start, end = oracle.build_grid(p, pos, "stable")
+    lam = oracle.lambda_(p, pos, idx, start, end)
+    vel = np.tile(np.array([[0.3, -0.2, 0.5, 0.0]], np.float32), (len(pos), 1))
+    out = oracle.xsph(p, pos, vel, lam, idx, start, end, precision="f64")
+    assert np.abs(out).max() == 0.0
+    # a linear field u = (z, 0, 0): XSPH pulls towards the neighbour average
+    vel[:, 0] = pos[:, 2]
+    out = oracle.xsph(p, pos, vel, lam, idx, start, end, precision="f64")
+    assert np.isfinite(out).all() and np.abs(out[:, 1:3]).max() == 0.0
+
+
+def test_finalize_clamp_and_pushout_against_analytic_box(fx, oracle):
+    cfg = _single(fx)
+    p = cfg.params(0.01)
+    p.n = 3
+    r = np.float32(cfg.particle_radius)
+    pred = np.array([[-1.5, 0.0, 1.0, 1], [0.0, 0.0, 2.5, 1], [0.05, 0.05, 1.05, 1]], np.float32)
+    vel = np.array([[-1.0, 0, 0, 0], [0, 0, 2.0, 0], [0, 0, -1.0, 0]], np.float32)
+    # a 20^3 collider (voxel 0.1, origin lo) with the single voxel (10, 10, 10) solid
+    col = np.zeros((20, 20, 20), np.uint8)
+    col[10, 10, 10] = 1
+    p.collider_dims[:] = [20, 20, 20]
+    p.collider_voxel = 0.1
+    pos, v = oracle.finalize(p, vel, np.zeros_like(vel), pred, col.reshape(-1), precision="f64")
+    assert pos[0, 0] == pytest.approx(-1.0 + r) and v[0, 0] == pytest.approx(0.35)
+    assert pos[1, 2] == pytest.approx(2.0 - r) and v[1, 2] == pytest.approx(-0.7)
+    # particle 2 sits at that voxel's centre: the six face neighbours are the
+    # nearest free centres (0.1 away); the scan (z outermost, strict <) keeps
+    # the first, dz = -1, so it is pushed down by 0.1 + r and its downward
+    # velocity (along the push) is kept
+    moved = pos[2, :3].astype(np.float64) - pred[2, :3]
+    assert moved[2] == pytest.approx(-(0.1 + r), rel=1e-5)
+    assert abs(moved[0]) < 1e-6 and abs(moved[1]) < 1e-6
+    assert v[2, 2] == pytest.approx(-1.0)
+
+
+def test_fp32_close_to_fp64(fx, oracle):
+    cfg, pos = _jittered_block(fx)
+    p = cfg.params(0.01)
+    keys, idx, start, end = oracle.build_grid(p, pos, "stable")
+    l32 = oracle.lambda_(p, pos, idx, start, end)
+    l64 = oracle.lambda_(p, pos, idx, start, end, precision="f64")
+    assert np.max(np.abs(l32[:, 0] - l64[:, 0]) / l64[:, 0]) < 1e-5
+
+
+def test_reference_update_overshoots_by_one_over_mass(fx, oracle):
+    """The reference's literal position correction (no particle mass) is the
+    mass-weighted one divided by m: the documented reason benchmark scenes use
+    FX_FLAG_MASS_WEIGHTED_DELTA."""
+    cfg, pos = _jittered_block(fx)
+    p = cfg.params(0.01)
+    keys, idx, start, end = oracle.build_grid(p, pos, "stable")
+    lam = oracle.lambda_(p, pos, idx, start, end, precision="f64")
+    lit = oracle.delta(p, pos, lam, idx, start, end, precision="f64")
+    p.flags = 1
+    mw = oracle.delta(p, pos, lam, idx, start, end, precision="f64")
+    np.testing.assert_allclose(mw[:, :3], lit[:, :3] * float(np.float32(p.mass)), rtol=1e-12, atol=1e-300)
+
+
+# ---- marching cubes (solver/marching_cubes.py) ---------------------------------
+
+def test_mc_tables_match_reference_golden(oracle):
+    tri, mask, walk = oracle.mc_tables()
+    assert np.array_equal(tri, GOLDEN["tri_table"])
+    assert np.array_equal(mask, GOLDEN["edge_table"])
+    assert np.array_equal(walk, GOLDEN["edge_walk"])
+
+
+@pytest.mark.parametrize("name", ["sphere", "noise", "random"])
+def test_mc_extract_matches_reference_golden(oracle, name):
+    g = GOLDEN
+    v, t = oracle.mc_extract(g[f"{name}_field"], g[f"{name}_dims"], g[f"{name}_origin"], float(g[f"{name}_spacing"]),
+                             float(g[f"{name}_iso"]))
+    assert np.array_equal(v, g[f"{name}_verts"]) and np.array_equal(t, g[f"{name}_tris"])
+
+
+def _watertight(tris):
+    from collections import Counter
+
+    c = Counter()
+    for a, b, d in tris:
+        for u, w in ((a, b), (b, d), (d, a)):
+            c[(min(u, w), max(u, w))] += 1
+    return all(v == 2 for v in c.values())
+
+
+def test_mc_properties_of_reference_tests(oracle):
+    """scripts/test_marching_cubes.py:33-200 on the oracle: every case covers
+    exactly its sign-change edges, sphere volume within 1%, watertight, and an
+    empty field extracts nothing."""
+    tri, mask, walk = oracle.mc_tables()
+    for case in range(256):
+        used = set(int(e) for e in tri[case] if e >= 0)
+        assert used == {e for e in range(12) if mask[case] & (1 << e)}
+    v, t = oracle.mc_extract(GOLDEN["sphere_field"], (41, 41, 41), (-1.0, -1.0, -1.0), 0.05, 0.0)
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    vol = np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0
+    assert abs(vol - 4 / 3 * math.pi * 0.7**3) / (4 / 3 * math.pi * 0.7**3) < 0.01 and vol > 0
+    assert _watertight(t)
+    v0, t0 = oracle.mc_extract(np.zeros(512, np.float32), (8, 8, 8), (0, 0, 0), 0.1, 0.5)
+    assert len(v0) == 0 and len(t0) == 0
