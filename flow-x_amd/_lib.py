@@ -145,30 +145,32 @@ SIGNATURES = {
 _lib = None
 
 
-def build(verbose=False):
+def build(verbose=False, out=None, defines=()):
     """Compile libflowx_hip.so for gfx950 in-tree (hipcc, no GPU needed)."""
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     sources = [str(CSRC / s) for s in HIP_SOURCES if (CSRC / s).exists()]
-    tmp = str(LIB_PATH) + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, f"-I{INCLUDE}", "-o", tmp, *sources]
+    target = Path(out) if out else LIB_PATH
+    tmp = str(target) + ".tmp"
+    cmd = [hipcc, *HIPCC_FLAGS, f"-I{INCLUDE}", *[f"-D{d}" for d in defines], "-o", tmp, *sources]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + res.stderr[-6000:])
     if verbose and res.stderr:
         print(res.stderr[-2000:])
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, target)
+    return target
 
 
 def lib():
     """The loaded library.  Raises if it has not been built (no fallback)."""
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
+        path = Path(os.environ.get("FLOWX_LIB", LIB_PATH))
+        if not path.exists():
             raise RuntimeError(
-                f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')"
+                f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')"
             )
-        L = ctypes.CDLL(str(LIB_PATH))
+        L = ctypes.CDLL(str(path))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name, None)
             if fn is None:

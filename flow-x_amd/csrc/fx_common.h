@@ -152,6 +152,39 @@ __device__ __forceinline__ void for_each_run(const Grid& g, const int32_t* __res
   }
 }
 
+// Scan slots [s, e) of a sorted-space array.  The loads of FX_BATCH
+// candidates are issued before any of them is used, so a wave keeps several
+// gathers in flight instead of waiting out each one (a load behind the
+// previous candidate's divergent branches is otherwise not hoisted).
+// refine(q, d) runs on every loaded candidate before any f(q, d) call and may
+// issue further (predicated) loads into d.
+#ifndef FX_BATCH
+#define FX_BATCH 4
+#endif
+struct NoRefine {
+  template <class D>
+  __device__ __forceinline__ void operator()(int, D&) const {}
+};
+
+template <class LD, class F, class RF = NoRefine>
+__device__ __forceinline__ void scan_run(int s, int e, LD&& ld, F&& f, RF&& refine = RF()) {
+  int q = s;
+  for (; q + FX_BATCH <= e; q += FX_BATCH) {
+    decltype(ld(q)) d[FX_BATCH];
+#pragma unroll
+    for (int u = 0; u < FX_BATCH; ++u) d[u] = ld(q + u);
+#pragma unroll
+    for (int u = 0; u < FX_BATCH; ++u) refine(q + u, d[u]);
+#pragma unroll
+    for (int u = 0; u < FX_BATCH; ++u) f(q + u, d[u]);
+  }
+  for (; q < e; ++q) {
+    auto d = ld(q);
+    refine(q, d);
+    f(q, d);
+  }
+}
+
 // ---- launch helpers ---------------------------------------------------------
 constexpr int kBlock = 256;
 

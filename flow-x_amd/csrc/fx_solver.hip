@@ -82,7 +82,7 @@ ProfScope::~ProfScope() {
   (void)hipEventRecord(g_prof_recs[slot].b, st);
 }
 
-static Tuning g_tuning = {0.2f, 256};
+static Tuning g_tuning = {0.0f, 256};
 
 Tuning tuning() { return g_tuning; }
 
@@ -522,11 +522,18 @@ struct Lists {
 
 enum { VISIT_FULL = 0, VISIT_BUILD = 1, VISIT_LIST = 2 };
 
-// f(q) is called for each candidate slot q in the reference's order and
-// returns whether the candidate is within the list radius (used by BUILD).
-template <int MODE, class F>
+struct PV {
+  float4 p, v;
+  float r2;
+};
+
+// ld(q) loads candidate slot q; f(q, data) is called for each candidate in
+// the reference's order and returns whether the candidate is within the list
+// radius (used by BUILD).
+template <int MODE, class LD, class F, class RF = NoRefine>
 __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__ off, int max_scan,
-                                      int bx, int by, int bz, int t, int n, const Lists& L, F&& f) {
+                                      int bx, int by, int bz, int t, int n, const Lists& L, LD&& ld,
+                                      F&& f, RF&& rf = RF()) {
   const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
   int full = 0;
   float di = 0.0f;
@@ -556,15 +563,20 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
         while (m) {
           const int b = __ffsll((unsigned long long)m) - 1;
           m &= m - 1;
-          f(s + b);
+          auto d = ld(s + b);
+          rf(s + b, d);
+          f(s + b, d);
         }
       } else if (MODE == VISIT_BUILD && e - s <= 64) {
         uint64_t m = 0;
-        for (int q = s; q < e; ++q)
-          if (f(q)) m |= 1ull << (q - s);
+        scan_run(
+            s, e, ld, [&](int q, const auto& d) {
+              if (f(q, d)) m |= 1ull << (q - s);
+            },
+            rf);
         L.mask[(size_t)r * n + t] = m;
       } else {
-        for (int q = s; q < e; ++q) f(q);
+        scan_run(s, e, ld, f, rf);
         if (MODE == VISIT_BUILD) {
           full |= 1 << r;
           L.mask[(size_t)r * n + t] = 0;
@@ -573,7 +585,7 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
     } else {
       for (int c = c0; c <= c1; ++c) {
         const int cs = off[c], ce = min(off[c + 1], cs + max_scan);
-        for (int q = cs; q < ce; ++q) f(q);
+        scan_run(cs, ce, ld, f, rf);
       }
       if (MODE == VISIT_BUILD) {
         full |= 1 << r;
@@ -590,17 +602,18 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
 
 // Run `body` over the candidates with the lists when they are valid for this
 // particle, else over the full stencil.
-template <int MODE, class F>
+template <int MODE, class LD, class F, class RF = NoRefine>
 __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restrict__ off, int max_scan,
-                                           int bx, int by, int bz, int t, int n, const Lists& L, F&& f) {
+                                           int bx, int by, int bz, int t, int n, const Lists& L, LD&& ld,
+                                           F&& f, RF&& rf = RF()) {
   if (MODE == VISIT_LIST) {
     const bool ok = L.mask && *L.built == 0 && L.cell[t] == cell_index(g, bx, by, bz);
     if (ok)
-      visit<VISIT_LIST>(g, off, max_scan, bx, by, bz, t, n, L, f);
+      visit<VISIT_LIST>(g, off, max_scan, bx, by, bz, t, n, L, ld, f, rf);
     else
-      visit<VISIT_FULL>(g, off, max_scan, bx, by, bz, t, n, L, f);
+      visit<VISIT_FULL>(g, off, max_scan, bx, by, bz, t, n, L, ld, f, rf);
   } else {
-    visit<MODE>(g, off, max_scan, bx, by, bz, t, n, L, f);
+    visit<MODE>(g, off, max_scan, bx, by, bz, t, n, L, ld, f, rf);
   }
 }
 
@@ -624,8 +637,8 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g, Consts k,
   {
 #pragma clang fp contract(fast)
     const float h = k.h, h2 = k.h2, r2l = L.r2_list;
-    visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, [&](int q) -> bool {
-      const float4 pj = pred_s[q];
+    auto ld = [&](int q) { return pred_s[q]; };
+    visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
       const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       const float r2 = dist2_exact(dx, dy, dz);
       if (r2 < h2) {
@@ -687,8 +700,8 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g, Consts k,
       const float h = k.h, h2 = k.h2;
       const bool scorr = k.scorr_k > 0.0f && k.w_q > 1e-9f;
       const float inv_wq = scorr ? 1.0f / k.w_q : 0.0f;
-      visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, [&](int q) -> bool {
-        const float4 pj = pred_in[q];
+      auto ld = [&](int q) { return pred_in[q]; };
+      visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2_exact(dx, dy, dz);
         if (r2 < h2 && r2 > 1e-12f && q != t) {
@@ -768,12 +781,22 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p, Grid g, Consts 
   {
 #pragma clang fp contract(fast)
     const float h2 = k.h2;
-    visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, [&](int q) -> bool {
-      const float4 pj = pred_s[q];
+    // stage 1 loads positions; stage 2 loads velocities only for candidates
+    // within h (predicated, still before any use); stage 3 accumulates
+    auto ld = [&](int q) {
+      PV c;
+      c.p = pred_s[q];
+      return c;
+    };
+    auto rf = [&](int q, PV& c) {
+      c.r2 = dist2_exact(pi.x - c.p.x, pi.y - c.p.y, pi.z - c.p.z);
+      if (c.r2 < h2 && q != t) c.v = vel_s[q];
+    };
+    visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
+      const float4 pj = c.p, vj = c.v;
       const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-      const float r2 = dist2_exact(dx, dy, dz);
+      const float r2 = c.r2;
       if (r2 < h2 && q != t) {
-        const float4 vj = vel_s[q];
         const float d = h2 - r2;
         const float w = d * d * d * __builtin_amdgcn_rcpf(fmaxf(vj.w, 1e-6f));
         ax += (vj.x - vi.x) * w;
@@ -781,7 +804,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p, Grid g, Consts 
         az += (vj.z - vi.z) * w;
       }
       return false;
-    });
+    }, rf);
   }
   const float f = k.visc * (k.mass * k.poly6);
   const float dvx = f * ax, dvy = f * ay, dvz = f * az;
@@ -818,8 +841,8 @@ __global__ __launch_bounds__(256) void k_normal(int n, Grid g, Consts k,
   {
 #pragma clang fp contract(fast)
     const float h = k.h, h2 = k.h2;
-    visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, [&](int q) -> bool {
-      const float4 pj = pred_s[q];
+    auto ld = [&](int q) { return pred_s[q]; };
+    visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
       const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       const float r2 = dist2_exact(dx, dy, dz);
       if (r2 < h2 && q != t) {
@@ -1116,7 +1139,8 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
       mark_grid(s.work, n, nc);
       if (n > 0) {
         FX_HIP_CHECK(hipMemcpyAsync(s.sorted_idx, w.sidx, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
-        LAUNCH(k_sorted_keys, n, st, n, w.sidx, w.key, s.sorted_key);
+        LAUNCH(k_sorted_keys, n, st, n, w.sidx, w.key, w.skey);
+        FX_HIP_CHECK(hipMemcpyAsync(s.sorted_key, w.skey, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
       }
       break;
     }

@@ -141,12 +141,22 @@ __constant__ McTables kMc = build_mc_tables();
 
 // ---- splat (shaders/surface_splat.glsl:32-97) ------------------------------
 // One thread per sample (x fastest), gathering the 5x5x5 cells around the
-// sample's clamped cell as 25 contiguous x-runs of sorted slots; pos_s holds
-// the finalized positions in the grid's sorted order.
+// sample's clamped cell as contiguous x-runs of sorted slots; pos_s holds the
+// finalized positions in the grid's sorted order.  Cells that cannot hold a
+// particle within the kernel radius are skipped: dmax bounds how far any
+// particle lies outside the box of the cell it was sorted into (the grid is
+// built from pre-constraint positions, shaders/surface_splat.glsl:8-9), so a
+// cell whose box is farther than kr + dmax contributes only zero terms.  The
+// visited terms and their order are the reference's, so the sum is exact.
+__device__ __forceinline__ float axis_gap(float p, float lo, float hi) {
+  return fmaxf(fmaxf(lo - p, p - hi), 0.0f);
+}
+
 __global__ __launch_bounds__(256) void k_splat(FxSurface sf, Grid g, float mass, float rho0,
                                                float h2, float coef, int max_scan, Collider col,
                                                const float4* __restrict__ pos_s,
                                                const int32_t* __restrict__ off,
+                                               const int32_t* __restrict__ dmax_bits, float kr,
                                                float* __restrict__ field, long long S) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
@@ -157,17 +167,50 @@ __global__ __launch_bounds__(256) void k_splat(FxSurface sf, Grid g, float mass,
   const float pz = sf.lo[2] + (float)gz * sf.spacing;
   int bx, by, bz;
   cell_coord(g, px, py, pz, bx, by, bz);
+  const float R = kr + __int_as_float(*dmax_bits) + 1e-3f * kr;
+  const float R2 = R * R;
   float acc = 0.0f;
-  {
-#pragma clang fp contract(fast)
-    for_each_run<2>(g, off, max_scan, bx, by, bz, [&](int a, int e) {
-      for (int q = a; q < e; ++q) {
-        const float4 pj = pos_s[q];
-        const float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
-        const float r2 = dist2_exact(dx, dy, dz);
-        if (r2 < h2) acc = poly6_accum_exact(acc, mass, coef, h2 - r2);
+  auto ld = [&](int q) { return pos_s[q]; };
+  auto body = [&](int, const float4& pj) {
+    const float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+    const float r2 = dist2_exact(dx, dy, dz);
+    if (r2 < h2) acc = poly6_accum_exact(acc, mass, coef, h2 - r2);
+  };
+  for (int dz = -2; dz <= 2; ++dz) {
+    const int z = bz + dz;
+    if (z < 0 || z >= g.cz) continue;
+    const float zlo = g.loz + (float)z * g.cell;
+    const float gzd = axis_gap(pz, zlo, zlo + g.cell);
+    const float rz = R2 - gzd * gzd;
+    if (rz <= 0.0f) continue;
+    for (int dy = -2; dy <= 2; ++dy) {
+      const int y = by + dy;
+      if (y < 0 || y >= g.cy) continue;
+      const float ylo = g.loy + (float)y * g.cell;
+      const float gyd = axis_gap(py, ylo, ylo + g.cell);
+      const float ryz = rz - gyd * gyd;
+      if (ryz <= 0.0f) continue;
+      int xa = g.cx, xb = -1;
+      for (int x = max(bx - 2, 0); x <= min(bx + 2, g.cx - 1); ++x) {
+        const float xlo = g.lox + (float)x * g.cell;
+        const float gxd = axis_gap(px, xlo, xlo + g.cell);
+        if (gxd * gxd < ryz) {
+          xa = min(xa, x);
+          xb = max(xb, x);
+        }
       }
-    });
+      if (xb < xa) continue;
+      const int c0 = (z * g.cy + y) * g.cx + xa, c1 = c0 + (xb - xa);
+      const int a = off[c0], e = off[c1 + 1];
+      if (e - a <= max_scan) {
+        scan_run(a, e, ld, body);
+      } else {
+        for (int c = c0; c <= c1; ++c) {
+          const int cs = off[c], ce = min(off[c + 1], cs + max_scan);
+          scan_run(cs, ce, ld, body);
+        }
+      }
+    }
   }
   float value = acc / fmaxf(rho0, 1e-6f);
   if (col.occ) {
@@ -178,12 +221,29 @@ __global__ __launch_bounds__(256) void k_splat(FxSurface sf, Grid g, float mass,
   field[s] = value;
 }
 
-__global__ void k_gather_pos(int n, const int32_t* __restrict__ sidx, const float4* __restrict__ pos,
-                             float4* __restrict__ pos_s) {
+// Gathers the finalized positions into sorted order and bounds how far any
+// particle lies outside the box of its cell (wave max, then one atomic).
+__global__ void k_gather_pos(int n, const int32_t* __restrict__ sidx, const int32_t* __restrict__ skey,
+                             Grid g, const float4* __restrict__ pos, float4* __restrict__ pos_s,
+                             int32_t* __restrict__ dmax_bits) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  pos_s[t] = pos[sidx[t]];
+  float d = 0.0f;
+  if (t < n) {
+    const float4 p = pos[sidx[t]];
+    pos_s[t] = p;
+    const int c = skey[t];
+    const int x = c % g.cx, y = (c / g.cx) % g.cy, z = c / (g.cx * g.cy);
+    const float xlo = g.lox + (float)x * g.cell, ylo = g.loy + (float)y * g.cell, zlo = g.loz + (float)z * g.cell;
+    const float ex = axis_gap(p.x, xlo, xlo + g.cell), ey = axis_gap(p.y, ylo, ylo + g.cell),
+                ez = axis_gap(p.z, zlo, zlo + g.cell);
+    d = sqrtf(ex * ex + ey * ey + ez * ez) * 1.000001f;
+  }
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) d = fmaxf(d, __shfl_xor(d, k, 64));
+  if ((threadIdx.x & 63) == 0 && d > 0.0f) atomicMax(dmax_bits, __float_as_int(d));
 }
+
+__global__ void k_zero_i32(int32_t* a) { *a = 0; }
 
 // ---- marching cubes -------------------------------------------------------
 constexpr int kMcBlock = 256;
@@ -429,16 +489,20 @@ int fx_splat(const FxParams* p, const FxSurface* surf, const FxState* s, float* 
   char* b = (char*)s->work;
   const int32_t* sidx = (const int32_t*)(b + wl.sidx);
   const int32_t* off = (const int32_t*)(b + wl.off);
+  const int32_t* skey = (const int32_t*)(b + wl.skey);
+  int32_t* dmax = (int32_t*)(b + wl.flags) + 3;
   float4* pos_s = (float4*)(b + wl.pos_s);
+  FX_LAUNCH(k_zero_i32, dim3(1), dim3(1), st, dmax);
   if (p->n > 0)
-    FX_LAUNCH(k_gather_pos, dim3(grid_for(p->n)), dim3(kBlock), st, p->n, sidx, (const float4*)s->pos, pos_s);
+    FX_LAUNCH(k_gather_pos, dim3(grid_for(p->n)), dim3(kBlock), st, p->n, sidx, skey, make_grid(*p),
+              (const float4*)s->pos, pos_s, dmax);
   const long long S = (long long)surf->dims[0] * surf->dims[1] * surf->dims[2];
   const float kr = surf->kernel_radius;
   const float coef = poly6_coef(kr);
   Collider col = make_collider(*p, s->collider);
-  FX_LAUNCH(k_splat, dim3(grid_for(S)), dim3(256), st, *surf, make_grid(*p), p->mass,
-                     p->rest_density, kr * kr, coef, p->max_cell_scan > 0 ? p->max_cell_scan : 512, col,
-                     (const float4*)pos_s, off, field, S);
+  FX_LAUNCH(k_splat, dim3(grid_for(S)), dim3(256), st, *surf, make_grid(*p), p->mass, p->rest_density,
+            kr * kr, coef, p->max_cell_scan > 0 ? p->max_cell_scan : 512, col, (const float4*)pos_s, off,
+            (const int32_t*)dmax, kr, field, S);
   return check_launch("fx_splat");
 }
 

@@ -377,6 +377,7 @@ def test_neighbour_lists_are_bit_identical_to_full_stencil(fx, scene, tuning):
     p, st, col = scene["cfg"].params(scene["dt"]), scene["st"], scene["col"]
     p.surface_tension = 0.001
     outs = []
+    saved = fx._lib.get_tuning()
     try:
         for skin, block in [(0.0, 256), tuning]:
             fx._lib.set_tuning(list_skin=skin, block=block)
@@ -386,6 +387,6 @@ def test_neighbour_lists_are_bit_identical_to_full_stencil(fx, scene, tuning):
                 g.run("substep")
             outs.append((g.get("pos"), g.get("vel"), g.get("lam"), g.get("normal")))
     finally:
-        fx._lib.set_tuning(list_skin=0.2, block=256)
+        fx._lib.set_tuning(**saved)
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
