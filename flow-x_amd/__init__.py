@@ -5,6 +5,7 @@ The directory is named flow-x_amd; load it as the package ``flowx_amd`` with
 ``__graft_entry__.load_package()`` (or add a spec loader of your own)."""
 
 from . import _lib
+from .batch import BatchSolver
 from .collision import ColliderGrid, grid_geometry
 from .config import (
     DomainSettings,
@@ -22,7 +23,7 @@ from .sph import Solver
 from .surface import MarchingCubes, Surface, SurfaceConfig, resolve
 
 __all__ = [
-    "ColliderGrid", "DomainSettings", "MarchingCubes", "Solver", "SolverConfig", "Surface", "SurfaceConfig",
+    "BatchSolver", "ColliderGrid", "DomainSettings", "MarchingCubes", "Solver", "SolverConfig", "Surface", "SurfaceConfig",
     "adjoint_config", "animated_config", "batch_configs", "dam_break_config", "grid_geometry",
     "lattice_config", "resolve", "resolve_config", "scale_config", "seed_positions",
 ]

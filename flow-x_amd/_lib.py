@@ -84,6 +84,10 @@ class FxTuning(ctypes.Structure):
     _fields_ = [("list_skin", ctypes.c_float), ("block", ctypes.c_int32)]
 
 
+class FxScene(ctypes.Structure):
+    _fields_ = [("params", FxParams), ("collider_offset", ctypes.c_int64)]
+
+
 class FxState(ctypes.Structure):
     _fields_ = [
         ("pos", ctypes.c_void_p),
@@ -134,9 +138,9 @@ SIGNATURES = {
     "fx_voxelize_cylinder": (
         _I, [_VP, _VP, ctypes.c_float, _VP, ctypes.c_float, ctypes.c_float, _VP, _I, _VP]
     ),
-    "fx_batch_workspace_bytes": (ctypes.c_size_t, [_VP, _I, _VP]),
-    "fx_batch_substep": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
-    "fx_batch_load_state": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
+    "fx_batch_workspace_bytes": (ctypes.c_size_t, [_VP, _I]),
+    "fx_batch_substep": (_I, [_VP, _I, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
+    "fx_batch_load_state": (_I, [_VP, _I, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
     "fx_adjoint_workspace_bytes": (ctypes.c_size_t, [_PP]),
     "fx_substep_record": (_I, [_PP, _SP, _VP, _VP, ctypes.c_size_t, _VP]),
     "fx_substep_adjoint": (_I, [_PP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),

@@ -491,6 +491,39 @@ __global__ void k_cell_range_ref(int n, int nc, const int32_t* __restrict__ skey
   if (key != next) end[key] = i + 1;
 }
 
+// ---- batched scenes -----------------------------------------------------------
+// Many independent scenes share one launch per pass: their particles are
+// concatenated (scene s owns particles [poff, poff + n)), their cells are
+// offset into one global key space (scene s owns keys [coff, coff + nc)), so
+// one counting sort orders every scene at once and scene s keeps the sorted
+// slots [poff, poff + n).  A thread finds its scene by binary search over the
+// device table and runs the plain pass with that scene's grid, constants and
+// cell offsets.
+struct SceneDev {
+  FxParams p;
+  Consts k;
+  Grid g;
+  Collider col;
+  int poff, n, coff, nc;
+};
+
+struct Scenes {
+  const SceneDev* s;
+  int count;
+};
+
+__device__ __forceinline__ int scene_of(const Scenes& S, int t) {
+  int lo = 0, hi = S.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (S.s[mid].poff <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 // ---- neighbour passes (sorted space) -------------------------------------
 //
 // Neighbour lists.  The first lambda pass of a substep visits the full
@@ -621,15 +654,26 @@ __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restr
 // pred_s[t].w = lambda (read by the delta pass with the position), and the
 // canonical lambda4 row if requested.  The density sum is bit exact
 // (dist2_exact / poly6_accum_exact in the reference's order).
-template <int MODE>
-__global__ __launch_bounds__(256) void k_lambda(int n, Grid g, Consts k,
+// V = MODE + 8 * BATCH
+template <int V>
+__global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
                                                 float4* __restrict__ pred_s,
-                                                const int32_t* __restrict__ off,
+                                                const int32_t* __restrict__ off_,
                                                 float2* __restrict__ dl_s,
                                                 const int32_t* __restrict__ sidx,
-                                                float4* __restrict__ lambda4, Lists L) {
+                                                float4* __restrict__ lambda4, Lists L, Scenes S) {
+  constexpr int MODE = V & 3;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  Grid g = g_;
+  Consts k = k_;
+  const int32_t* off = off_;
+  if (V & 8) {
+    const SceneDev& sc = S.s[scene_of(S, t)];
+    g = sc.g;
+    k = sc.k;
+    off = off_ + sc.coff;
+  }
   const float4 pi = pred_s[t];
   int bx, by, bz;
   cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
@@ -677,19 +721,29 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g, Consts k,
 // iteration, shaders/sph_velocity.glsl:12-24 into vel_s (w = own density).
 // delta4 (canonical, optional) receives the correction itself.  With lists,
 // each particle's accumulated |dp| feeds the global displacement bound.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_delta(int n, Grid g, Consts k,
+template <int V>
+__global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
                                                const float4* __restrict__ pred_in,
                                                const float2* __restrict__ dl_s,
-                                               const int32_t* __restrict__ off,
+                                               const int32_t* __restrict__ off_,
                                                float4* __restrict__ pred_out,
                                                const float4* __restrict__ pos_s,
                                                float4* __restrict__ vel_s,
                                                const int32_t* __restrict__ sidx,
                                                float4* __restrict__ delta4, Lists L,
-                                               const int32_t* __restrict__ skey) {
+                                               const int32_t* __restrict__ skey, Scenes S) {
+  constexpr int MODE = V & 3;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) {
+    Grid g = g_;
+    Consts k = k_;
+    const int32_t* off = off_;
+    if (V & 8) {
+      const SceneDev& sc = S.s[scene_of(S, t)];
+      g = sc.g;
+      k = sc.k;
+      off = off_ + sc.coff;
+    }
     const float4 pi = pred_in[t];
     const float li = pi.w;
     int bx, by, bz;
@@ -760,19 +814,32 @@ __global__ void k_velocity_s(int n, float dt, const float4* __restrict__ pred_s,
 // (plus pos_s for the splat); otherwise write the correction to delta4 only.
 // V = 4 * FUSE + MODE (one template argument keeps launch macros simple)
 template <int V>
-__global__ __launch_bounds__(256) void k_xsph(int n, FxParams p, Grid g, Consts k,
-                                              Collider col, const float4* __restrict__ pred_s,
+__global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Consts k_,
+                                              Collider col_, const float4* __restrict__ pred_s,
                                               const float4* __restrict__ vel_s,
-                                              const int32_t* __restrict__ off,
+                                              const int32_t* __restrict__ off_,
                                               const int32_t* __restrict__ sidx,
                                               float4* __restrict__ delta4,
                                               float4* __restrict__ pos4, float4* __restrict__ vel4,
                                               float4* __restrict__ pos_s,
-                                              float4* __restrict__ pred4_out, Lists L) {
+                                              float4* __restrict__ pred4_out, Lists L, Scenes S) {
   constexpr bool FUSE = (V & 4) != 0;
   constexpr int MODE = V & 3;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  FxParams p = p_;
+  Grid g = g_;
+  Consts k = k_;
+  Collider col = col_;
+  const int32_t* off = off_;
+  if (V & 8) {
+    const SceneDev& sc = S.s[scene_of(S, t)];
+    p = sc.p;
+    g = sc.g;
+    k = sc.k;
+    col = sc.col;
+    off = off_ + sc.coff;
+  }
   const float4 pi = pred_s[t];
   const float4 vi = vel_s[t];
   int bx, by, bz;
@@ -826,14 +893,24 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p, Grid g, Consts 
 
 // shaders/sph_normal.glsl:24-81 over the previous grid; pred_s.w = the
 // neighbour's density from the last lambda pass.  Writes canonical normal4.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_normal(int n, Grid g, Consts k,
+template <int V>
+__global__ __launch_bounds__(256) void k_normal(int n, Grid g_, Consts k_,
                                                 const float4* __restrict__ pred_s,
-                                                const int32_t* __restrict__ off,
+                                                const int32_t* __restrict__ off_,
                                                 const int32_t* __restrict__ sidx,
-                                                float4* __restrict__ normal4, Lists L) {
+                                                float4* __restrict__ normal4, Lists L, Scenes S) {
+  constexpr int MODE = V & 3;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  Grid g = g_;
+  Consts k = k_;
+  const int32_t* off = off_;
+  if (V & 8) {
+    const SceneDev& sc = S.s[scene_of(S, t)];
+    g = sc.g;
+    k = sc.k;
+    off = off_ + sc.coff;
+  }
   const float4 pi = pred_s[t];
   int bx, by, bz;
   cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
@@ -866,6 +943,49 @@ __global__ __launch_bounds__(256) void k_normal(int n, Grid g, Consts k,
   const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
   const float curvature = (mag > 1e-4f) ? -lap / mag : 0.0f;
   normal4[min(max(sidx[t], 0), n - 1)] = make_float4(nx, ny, nz, curvature);
+}
+
+
+// Batched fused predict + grid key + histogram: particle i belongs to the
+// scene whose particle range holds it; keys are offset by the scene's cells.
+__global__ __launch_bounds__(256) void k_b_predict_key(int ntot, Scenes S, const float4* __restrict__ pos,
+                                                       float4* __restrict__ vel,
+                                                       const float4* __restrict__ nrm,
+                                                       float4* __restrict__ pred,
+                                                       int32_t* __restrict__ key,
+                                                       int32_t* __restrict__ cnt,
+                                                       int32_t* __restrict__ rank, int do_predict) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntot) return;
+  const SceneDev& sc = S.s[scene_of(S, i)];
+  float4 pp;
+  if (do_predict) {
+    float4 vo;
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    predict_one(sc.p, pos[i], vel[i], (sc.p.surface_tension > 0.0f && nrm) ? nrm[i] : zero, vo, pp);
+    vel[i] = vo;
+    pred[i] = pp;
+  } else {
+    pp = pos[i];
+    pp.w = 1.0f;
+    pred[i] = pp;
+  }
+  int x, y, z;
+  cell_coord(sc.g, pp.x, pp.y, pp.z, x, y, z);
+  const int k = sc.coff + cell_index(sc.g, x, y, z);
+  key[i] = k;
+  rank[i] = atomicAdd(&cnt[k], 1);
+}
+
+__global__ void k_b_load_state(int ntot, float4* __restrict__ pos, float4* __restrict__ vel) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntot) return;
+  float4 p = pos[i];
+  p.w = 1.0f;
+  pos[i] = p;
+  float4 v = vel[i];
+  v.w = 0.0f;
+  vel[i] = v;
 }
 
 // ---- canonical streaming passes (FX_PASS_*) --------------------------------
@@ -1048,7 +1168,7 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
     if (!s.normal) return fail(FX_EINVAL, "surface_tension > 0 needs a normal buffer");
     if (!grid_ready(s.work, n, p.n_cells))
       return fail(FX_EINVAL, "surface tension reads the previous grid: call fx_load_state first");
-    if (n > 0) NLAUNCH(k_normal<VISIT_LIST>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L);
+    if (n > 0) NLAUNCH(k_normal<VISIT_LIST>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L, Scenes{nullptr, 0});
   }
   const int iters = p.iterations;
   int cur = iters & 1;  // the last iteration writes buffer 0
@@ -1062,19 +1182,20 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
     const bool last = it == iters - 1;
     float4* lam_out = (last && s.lambda) ? (float4*)s.lambda : (float4*)nullptr;
     if (!lists)
-      NLAUNCH(k_lambda<VISIT_FULL>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L);
+      NLAUNCH(k_lambda<VISIT_FULL>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else if (it == 0)
-      NLAUNCH(k_lambda<VISIT_BUILD>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L);
+      NLAUNCH(k_lambda<VISIT_BUILD>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else
-      NLAUNCH(k_lambda<VISIT_LIST>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L);
+      NLAUNCH(k_lambda<VISIT_LIST>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     NLAUNCH(k_delta<VISIT_LIST>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
             w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
-            (float4*)nullptr, L, lists ? (const int32_t*)w.skey : (const int32_t*)nullptr);
+            (float4*)nullptr, L, lists ? (const int32_t*)w.skey : (const int32_t*)nullptr, Scenes{nullptr, 0});
     cur ^= 1;
   }
   if (iters == 0) LAUNCH(k_velocity_s, n, st, n, p.dt, w.pred_s[0], w.pos_s, w.dl_s, w.vel_s);
   NLAUNCH(k_xsph<4 + VISIT_LIST>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
-          w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred, L);
+          w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred, L,
+          Scenes{nullptr, 0});
   return check_launch("substep");
 }
 
@@ -1164,7 +1285,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
         LAUNCH(k_gather, n, st, n, w.sidx, (const int32_t*)nullptr, (const float4*)s.pred, (const float4*)nullptr,
                (const float4*)nullptr, w.pred_s[0], (float4*)nullptr, (float2*)nullptr,
                (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
-        NLAUNCH(k_lambda<VISIT_FULL>, n, st, n, g, k, w.pred_s[0], w.off, w.dl_s, w.sidx, (float4*)s.lambda, L0);
+        NLAUNCH(k_lambda<VISIT_FULL>, n, st, n, g, k, w.pred_s[0], w.off, w.dl_s, w.sidx, (float4*)s.lambda, L0, Scenes{nullptr, 0});
       }
       break;
     case FX_PASS_DELTA:
@@ -1180,7 +1301,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
         LAUNCH(k_w_from_lambda, n, st, n, w.pred_s[0], (const float2*)w.dl_s);
         NLAUNCH(k_delta<VISIT_FULL>, n, st, n, g, k, (const float4*)w.pred_s[0], (const float2*)w.dl_s, w.off,
                 (float4*)nullptr, (const float4*)nullptr, (float4*)nullptr, w.sidx, (float4*)s.delta, L0,
-                (const int32_t*)nullptr);
+                (const int32_t*)nullptr, Scenes{nullptr, 0});
       }
       break;
     case FX_PASS_APPLY_DELTA:
@@ -1205,7 +1326,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
                (const float4*)s.lambda, w.pred_s[0], w.vel_s);
         NLAUNCH(k_xsph<VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
                 (const float4*)w.vel_s, w.off, w.sidx, (float4*)s.delta, (float4*)nullptr,
-                (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, L0);
+                (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, L0, Scenes{nullptr, 0});
       }
       break;
     case FX_PASS_FINALIZE:
@@ -1226,7 +1347,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
         LAUNCH(k_gather, n, st, n, w.sidx, (const int32_t*)nullptr, (const float4*)s.pred, (const float4*)nullptr,
                (const float4*)s.lambda, w.pred_s[0], (float4*)nullptr, (float2*)nullptr,
                (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
-        NLAUNCH(k_normal<VISIT_FULL>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L0);
+        NLAUNCH(k_normal<VISIT_FULL>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L0, Scenes{nullptr, 0});
       }
       break;
     default:
@@ -1235,6 +1356,129 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
   return check_launch("fx_pass");
 }
 
+
+// ---- batched scenes: host side ----------------------------------------------
+struct BatchPlan {
+  long long ntot, ctot;
+  int iterations;
+  bool tension;
+  FxParams virt;  // n = ntot, n_cells = ctot: sizes the shared workspace
+  size_t table_off, normal_off, pred_off, solver_off, total;
+};
+
+static int batch_plan(const FxScene* scenes, int count, BatchPlan& b) {
+  if (!scenes || count < 1) return fail(FX_EINVAL, "batch: need at least one scene");
+  b.ntot = 0;
+  b.ctot = 0;
+  b.tension = false;
+  b.iterations = scenes[0].params.iterations;
+  for (int i = 0; i < count; ++i) {
+    const FxParams& p = scenes[i].params;
+    FxState dummy{};
+    FX_TRY(validate(&p, &dummy, false));
+    if (p.iterations != b.iterations)
+      return fail(FX_EINVAL, "batch: scene %d has %d iterations, scene 0 has %d (must match)", i,
+                  p.iterations, b.iterations);
+    if (p.surface_tension > 0.0f) b.tension = true;
+    b.ntot += p.n;
+    b.ctot += p.n_cells;
+  }
+  if (b.ntot >= (1LL << 31) - 1 || b.ctot >= (1LL << 30))
+    return fail(FX_ERANGE, "batch: %lld particles / %lld cells exceed the index range", b.ntot, b.ctot);
+  b.virt = scenes[0].params;
+  b.virt.n = (int)b.ntot;
+  b.virt.n_cells = (int)b.ctot;
+  size_t o = 0;
+  b.table_off = o;
+  o += align256((size_t)count * sizeof(SceneDev));
+  b.normal_off = o;
+  o += align256((size_t)b.ntot * 16 + 16);
+  b.pred_off = o;
+  o += align256((size_t)b.ntot * 16 + 16);
+  b.solver_off = o;
+  o += workspace_layout(b.virt).total;
+  b.total = o;
+  return FX_OK;
+}
+
+static std::vector<SceneDev> scene_table(const FxScene* scenes, int count, const uint8_t* colliders) {
+  std::vector<SceneDev> t(count);
+  int poff = 0, coff = 0;
+  for (int i = 0; i < count; ++i) {
+    const FxParams& p = scenes[i].params;
+    SceneDev& d = t[i];
+    d.p = p;
+    d.k = make_consts(p);
+    d.g = make_grid(p);
+    const uint8_t* occ = (colliders && scenes[i].collider_offset >= 0) ? colliders + scenes[i].collider_offset : nullptr;
+    d.col = make_collider(p, occ);
+    d.poff = poff;
+    d.n = p.n;
+    d.coff = coff;
+    d.nc = p.n_cells;
+    poff += p.n;
+    coff += p.n_cells;
+  }
+  return t;
+}
+
+static int batch_step(const FxScene* scenes, int count, float* pos, float* vel, const uint8_t* colliders,
+                      void* work, size_t work_bytes, hipStream_t st, bool load) {
+  BatchPlan b;
+  FX_TRY(batch_plan(scenes, count, b));
+  if (!work || work_bytes < b.total)
+    return fail(FX_ENOMEM, "batch workspace too small: %zu < %zu bytes", work_bytes, b.total);
+  if (b.ntot > 0 && (!pos || !vel)) return fail(FX_EINVAL, "batch: pos and vel are required");
+  char* base = (char*)work;
+  SceneDev* table = (SceneDev*)(base + b.table_off);
+  float4* normal = (float4*)(base + b.normal_off);
+  float4* pred = (float4*)(base + b.pred_off);
+  void* wsp = base + b.solver_off;
+  Ws w = bind_ws(workspace_layout(b.virt), wsp);
+  Scenes S{table, count};
+  const int n = (int)b.ntot, nc = (int)b.ctot;
+  if (load) {
+    // the per-scene table lives in the workspace so captured substeps read it
+    // from device memory; this upload is the only host-to-device copy
+    std::vector<SceneDev> host = scene_table(scenes, count, colliders);
+    FX_HIP_CHECK(hipMemcpyAsync(table, host.data(), host.size() * sizeof(SceneDev), hipMemcpyHostToDevice, st));
+    FX_HIP_CHECK(hipStreamSynchronize(st));
+    if (n > 0) LAUNCH(k_b_load_state, n, st, n, (float4*)pos, (float4*)vel);
+  }
+  const Lists L0 = make_lists(w, make_consts(scenes[0].params), false);
+  if (!load && b.tension) {
+    if (!grid_ready(work, n, nc)) return fail(FX_EINVAL, "batch: surface tension needs fx_batch_load_state first");
+    NLAUNCH(k_normal<8 + VISIT_FULL>, n, st, n, Grid{}, Consts{}, w.pred_s[0], w.off, w.sidx, normal, L0, S);
+  }
+  const int iters = b.iterations;
+  const int cur0 = load ? 0 : (iters & 1);
+  FX_TRY(fill(w.cnt, nc + 1, 0, st));
+  if (n > 0)
+    LAUNCH(k_b_predict_key, n, st, n, S, (const float4*)pos, (float4*)vel, (const float4*)normal, pred, w.key,
+           w.cnt, w.rank, load ? 0 : 1);
+  FxState fs{};
+  FX_TRY(sort_from_counts(b.virt, fs, w, st));
+  if (n > 0)
+    LAUNCH(k_gather, n, st, n, w.sidx, w.key, (const float4*)pred, (const float4*)pos, (const float4*)nullptr,
+           w.pred_s[cur0], w.pos_s, w.dl_s, (int32_t*)nullptr, w.skey, (int32_t*)nullptr);
+  mark_grid(work, n, nc);
+  if (load || n == 0) return check_launch("batch");
+  int cur = cur0;
+  for (int it = 0; it < iters; ++it) {
+    const bool last = it == iters - 1;
+    NLAUNCH(k_lambda<8 + VISIT_FULL>, n, st, n, Grid{}, Consts{}, w.pred_s[cur], w.off, w.dl_s, w.sidx,
+            (float4*)nullptr, L0, S);
+    NLAUNCH(k_delta<8 + VISIT_FULL>, n, st, n, Grid{}, Consts{}, (const float4*)w.pred_s[cur],
+            (const float2*)w.dl_s, w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s,
+            last ? w.vel_s : (float4*)nullptr, w.sidx, (float4*)nullptr, L0, (const int32_t*)nullptr, S);
+    cur ^= 1;
+  }
+  if (iters == 0) LAUNCH(k_velocity_s, n, st, n, scenes[0].params.dt, w.pred_s[0], w.pos_s, w.dl_s, w.vel_s);
+  NLAUNCH(k_xsph<8 + 4 + VISIT_FULL>, n, st, n, FxParams{}, Grid{}, Consts{}, Collider{}, (const float4*)w.pred_s[0],
+          (const float4*)w.vel_s, w.off, w.sidx, (float4*)nullptr, (float4*)pos, (float4*)vel, w.pos_s,
+          (float4*)nullptr, L0, S);
+  return check_launch("batch");
+}
 }  // namespace fx
 
 // ---- C ABI -------------------------------------------------------------------
@@ -1325,6 +1569,22 @@ int fx_step(const FxParams* p, FxState* s, int substeps, void* stream) {
   if (substeps < 0) return fail(FX_EINVAL, "substeps < 0");
   for (int i = 0; i < substeps; ++i) FX_TRY(fx_substep(p, s, stream));
   return FX_OK;
+}
+
+size_t fx_batch_workspace_bytes(const FxScene* scenes, int count) {
+  BatchPlan b;
+  if (batch_plan(scenes, count, b) != FX_OK) return 0;
+  return b.total;
+}
+
+int fx_batch_load_state(const FxScene* scenes, int count, float* pos, float* vel, const uint8_t* colliders,
+                        void* work, size_t work_bytes, void* stream) {
+  return batch_step(scenes, count, pos, vel, colliders, work, work_bytes, (hipStream_t)stream, true);
+}
+
+int fx_batch_substep(const FxScene* scenes, int count, float* pos, float* vel, const uint8_t* colliders,
+                     void* work, size_t work_bytes, void* stream) {
+  return batch_step(scenes, count, pos, vel, colliders, work, work_bytes, (hipStream_t)stream, false);
 }
 
 int fx_load_state(const FxParams* p, FxState* s, void* stream) {

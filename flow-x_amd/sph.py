@@ -150,6 +150,26 @@ class Solver:
         _lib.check(_lib.lib().fx_step(ctypes.byref(self.params_for(dt)), ctypes.byref(self.state),
                                       int(count), self._stream()), "fx_step")
 
+    def capture(self, dt, substeps):
+        """Capture `substeps` substeps of length dt in a HIP graph and return
+        it; graph.replay() then runs them with one host call.  One warm-up
+        substep runs eagerly first (capture itself executes nothing), so the
+        state advances by one substep here."""
+        self._require_seeded()
+        self.last_dt = dt
+        torch.cuda.synchronize(self.device)
+        side = torch.cuda.Stream(self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            self.substep(dt)
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        torch.cuda.synchronize(self.device)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for _ in range(substeps):
+                self.substep(dt)
+        return graph
+
     def step(self, frame_dt):
         """_step (solver/sph.py:517-545) without the Blender outputs: the
         frame's substeps, timed end to end (the wall time includes the GPU

@@ -196,6 +196,27 @@ int fx_step(const FxParams* p, FxState* s, int substeps, void* stream);
  * from s->pos / s->vel. */
 int fx_load_state(const FxParams* p, FxState* s, void* stream);
 
+/* ---- batched scenes (no reference counterpart; the reference runs one
+ * scene, solver/sph.py:501-514).  Many independent scenes advance in one
+ * launch per pass: particles are concatenated in scene order (scene i owns
+ * the next scenes[i].params.n rows of pos / vel), cells are offset into one
+ * key space, one counting sort orders every scene, and each thread reads its
+ * scene's parameters from a table the library keeps in `work`.  Every scene
+ * must use the same iteration count.  fx_batch_load_state uploads the table
+ * (synchronously) and builds the grid from pos (pred = pos); call it again
+ * whenever a scene's parameters change.  fx_batch_substep reads only device
+ * memory and may be captured in a HIP graph. */
+typedef struct FxScene {
+  FxParams params;           /* the scene's parameters; params.n = its particles  */
+  int64_t collider_offset;   /* first byte of its grid in `colliders`, -1 = none  */
+} FxScene;
+
+size_t fx_batch_workspace_bytes(const FxScene* scenes, int count);
+int fx_batch_load_state(const FxScene* scenes, int count, float* pos, float* vel,
+                        const uint8_t* colliders, void* work, size_t work_bytes, void* stream);
+int fx_batch_substep(const FxScene* scenes, int count, float* pos, float* vel,
+                     const uint8_t* colliders, void* work, size_t work_bytes, void* stream);
+
 /* ---- surface (solver/surface.py:201-251 update, shaders/surface_splat.glsl)
  * Density field at every surface sample, gathered through the last grid from
  * the finalized positions s->pos (x fastest, S = dims product). */
