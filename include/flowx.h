@@ -217,6 +217,36 @@ int fx_batch_load_state(const FxScene* scenes, int count, float* pos, float* vel
 int fx_batch_substep(const FxScene* scenes, int count, float* pos, float* vel,
                      const uint8_t* colliders, void* work, size_t work_bytes, void* stream);
 
+/* ---- differentiable substep (no reference counterpart) -------------------
+ * A substep whose neighbour structure is recorded once after predict (pairs
+ * within h*(1 + list_skin) of the predicted positions, each list sorted by
+ * index) and whose collider voxel frame moves with `translation`, in float
+ * (precision 4) or double (precision 8).  State arrays are N x 4 of that
+ * precision (x, y, z, unused).  fx_adjoint_backward recomputes the substep
+ * from its checkpoint (x, v), keeping every Jacobi iterate, and writes the
+ * adjoints of x and v given those of the outputs; grad_params (device
+ * double[4]) accumulates d/dgravity and d/dtranslation.  surface_tension and
+ * scorr_k must be 0.  fx_adjoint_overflow reports whether a neighbour list
+ * exceeded its 192 entries in the last forward. */
+size_t fx_adjoint_workspace_bytes(const FxParams* p, int precision);
+int fx_adjoint_forward(const FxParams* p, int precision, double gravity, const double translation[3],
+                       double list_skin, const uint8_t* collider, const void* x, const void* v,
+                       void* x_out, void* v_out, void* work, size_t work_bytes, void* stream);
+int fx_adjoint_backward(const FxParams* p, int precision, double gravity, const double translation[3],
+                        double list_skin, const uint8_t* collider, const void* x, const void* v,
+                        const void* gx_out, const void* gv_out, void* gx, void* gv,
+                        double* grad_params, void* work, size_t work_bytes, void* stream);
+int fx_adjoint_overflow(const FxParams* p, int precision, const void* work, int32_t* host_flag,
+                        void* stream);
+/* Splat of the density field (shaders/surface_splat.glsl arithmetic, sample
+ * zeroed where sample_mask is set) at the given precision, and its adjoint:
+ * gx += d field / d x ^T gfield. */
+int fx_adjoint_splat(const FxParams* p, const FxSurface* surf, int precision, const void* x,
+                     const uint8_t* sample_mask, void* field, void* work, size_t work_bytes,
+                     void* stream);
+int fx_adjoint_splat_backward(const FxParams* p, const FxSurface* surf, int precision, const void* x,
+                              const uint8_t* sample_mask, const void* gfield, void* gx, void* stream);
+
 /* ---- surface (solver/surface.py:201-251 update, shaders/surface_splat.glsl)
  * Density field at every surface sample, gathered through the last grid from
  * the finalized positions s->pos (x fastest, S = dims product). */
