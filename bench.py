@@ -152,6 +152,105 @@ def batch_workload(fx, steps, warmup, count=64):
     }
 
 
+def adjoint_workload(fx, substeps, directions, fd_checks=((4, 1e-4, 1e-8), (8, 1e-6, 1e-9))):
+    """200k particles (h = 0.05, 80x80x31, 4 iterations, 0.4 m box): forward
+    and backward of a `substeps` rollout in float32 (adjoint / forward cost),
+    then float64 central finite differences of the loss along `directions`
+    random initial-velocity directions, gravity and the three
+    collider-translation axes, for each (substeps, eps, eps_translation) of
+    `fd_checks`.  Translation moves every pushed-out particle (thousands per
+    substep) by eps at once, so its difference quotient leaves the linear
+    range at a far smaller step than a unit velocity direction spread over
+    all particles; it gets its own eps.  The perturbed runs replay the base
+    run's record (branches and neighbour lists).  The float64 gradient norm |dL/dv0| over growing horizons
+    measures how fast the substep map amplifies perturbations."""
+    import numpy as np
+    import torch
+
+    from flowx_amd.adjoint import DiffSolver, gaussian_target, loss_fn, rollout
+
+    out = {}
+    rng = np.random.default_rng(0)
+    centre = (float(rng.uniform(-0.5, 0.5)), float(rng.uniform(-0.5, 0.5)), float(rng.uniform(0.1, 0.9)))
+    for dtype in (torch.float32, torch.float64):
+        d = DiffSolver(fx.adjoint_config(), dtype=dtype)
+        x0 = torch.as_tensor(fx.seed_positions(d.cfg), dtype=dtype).cuda()
+        x0[:, 3] = 0
+        v0 = torch.zeros_like(x0)
+        g = torch.tensor(-9.81, dtype=torch.float64, device="cuda", requires_grad=True)
+        tr = torch.zeros(3, dtype=torch.float64, device="cuda", requires_grad=True)
+        target = gaussian_target(d, centre, sigma=0.2)
+        com_t = torch.tensor(centre, dtype=dtype, device="cuda")
+        if dtype == torch.float32:
+            v = v0.clone().requires_grad_(True)
+            rollout(d, x0, v, g, tr, 2, [], 1)  # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, _ = rollout(d, x0, v, g, tr, substeps, [], 1, check_overflow=False)
+            L = loss_fn(d, x, target, com_t)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            L.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            out.update(particles=len(x0), substeps=substeps, forward_s=round(t1 - t0, 4),
+                       backward_s=round(t2 - t1, 4), adjoint_over_forward=round((t2 - t1) / (t1 - t0), 3),
+                       list_overflow=d.overflowed(), fp32_grad_v0_norm=float(v.grad.norm()))
+            del x, L, v
+            torch.cuda.empty_cache()
+            continue
+        growth = []
+        for k in (1, 4, 8, 16, 30, substeps):
+            v = v0.clone().requires_grad_(True)
+            x, _ = rollout(d, x0, v, g, tr, k, [], 1)
+            loss_fn(d, x, target, com_t).backward()
+            growth.append((k, float(v.grad.norm())))
+            g.grad = tr.grad = None
+        out["fp64_grad_v0_norm_by_substeps"] = growth
+        checks = []
+        for fsteps, eps, eps_t in fd_checks:
+            g.grad = tr.grad = None
+            v = v0.clone().requires_grad_(True)
+            decisions = []  # the base run's record, replayed by every perturbed run
+            x, _ = rollout(d, x0, v, g, tr, fsteps, decisions, 1)
+            L = loss_fn(d, x, target, com_t)
+            L.backward()
+            gv, gg, gt = v.grad.clone(), g.grad.item(), tr.grad.cpu().numpy()
+
+            def L_at(dv=None, dg=0.0, dtr=None):
+                with torch.no_grad():
+                    vv = v0 + (dv if dv is not None else 0)
+                    tt = tr.detach() + (torch.as_tensor(dtr, dtype=torch.float64, device="cuda")
+                                        if dtr is not None else 0)
+                    xx, _ = rollout(d, x0, vv, g.detach() + dg, tt, fsteps, decisions, 2)
+                    return loss_fn(d, xx, target, com_t).item()
+
+            errs = []
+            for _ in range(directions):
+                dirn = torch.as_tensor(rng.normal(0, 1, v0.shape), dtype=dtype).cuda()
+                dirn[:, 3] = 0
+                dirn /= dirn.norm()
+                fd = (L_at(dv=eps * dirn) - L_at(dv=-eps * dirn)) / (2 * eps)
+                errs.append(abs(fd - float((gv * dirn).sum())) / max(abs(fd), 1e-300))
+            fdg = (L_at(dg=eps) - L_at(dg=-eps)) / (2 * eps)
+            fdt = []
+            for a in range(3):
+                e = np.zeros(3)
+                e[a] = eps_t
+                fdt.append((L_at(dtr=e) - L_at(dtr=-e)) / (2 * eps_t))
+            checks.append(dict(
+                substeps=fsteps, eps=eps, eps_translation=eps_t, directions=directions,
+                velocity_max_rel=float(max(errs)), velocity_median_rel=float(np.median(errs)),
+                gravity=(fdg, gg), gravity_rel=abs(fdg - gg) / abs(fdg),
+                translation=[(fdt[a], float(gt[a])) for a in range(3)],
+                translation_rel=[abs(fdt[a] - gt[a]) / max(abs(fdt[a]), 1e-300) for a in range(3)]))
+            del decisions
+        out["fd_checks"] = checks
+    return {"metric": "adjoint/forward cost, fp64 finite-difference agreement", "config": {
+        "workload": f"{out.get('particles')} particles, h=0.05, 4 iterations, 0.4 m box, {substeps} substeps"},
+        **out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,7 +261,10 @@ def main():
     ap.add_argument("--literal-update", action="store_true",
                     help="the reference's literal position correction (diverges at these masses)")
     ap.add_argument("--no-surface", action="store_true")
-    ap.add_argument("--workload", default="scale", choices=["scale", "batch"])
+    ap.add_argument("--workload", default="scale", choices=["scale", "batch", "adjoint"])
+    ap.add_argument("--fd-directions", type=int, default=32)
+    ap.add_argument("--fd-checks", default="4:1e-4:1e-8,8:1e-6:1e-9",
+                    help="adjoint workload: comma-separated substeps:eps:eps_translation finite-difference checks")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -178,6 +280,10 @@ def main():
 
     if args.workload == "batch":
         print(json.dumps(batch_workload(fx, args.steps, args.warmup)))
+        return
+    if args.workload == "adjoint":
+        checks = [(int(a), float(b), float(t)) for a, b, t in (c.split(":") for c in args.fd_checks.split(","))]
+        print(json.dumps(adjoint_workload(fx, args.steps, args.fd_directions, checks)))
         return
 
     cfg = fx.scale_config(literal_update=args.literal_update)

@@ -142,10 +142,11 @@ SIGNATURES = {
     "fx_batch_substep": (_I, [_VP, _I, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
     "fx_batch_load_state": (_I, [_VP, _I, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
     "fx_adjoint_workspace_bytes": (ctypes.c_size_t, [_PP, _I]),
-    "fx_adjoint_forward": (_I, [_PP, _I, ctypes.c_double, _VP, ctypes.c_double, _VP, _VP, _VP, _VP, _VP, _VP,
-                                ctypes.c_size_t, _VP]),
+    "fx_adjoint_record_bytes": (ctypes.c_size_t, [_PP]),
+    "fx_adjoint_forward": (_I, [_PP, _I, ctypes.c_double, _VP, ctypes.c_double, _VP, _VP, _I, _VP, _VP, _VP, _VP,
+                                _VP, ctypes.c_size_t, _VP]),
     "fx_adjoint_backward": (_I, [_PP, _I, ctypes.c_double, _VP, ctypes.c_double, _VP, _VP, _VP, _VP, _VP, _VP,
-                                 _VP, _VP, _VP, ctypes.c_size_t, _VP]),
+                                 _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
     "fx_adjoint_overflow": (_I, [_PP, _I, _VP, _VP, _VP]),
     "fx_adjoint_splat": (_I, [_PP, ctypes.POINTER(FxSurface), _I, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
     "fx_adjoint_splat_backward": (_I, [_PP, ctypes.POINTER(FxSurface), _I, _VP, _VP, _VP, _VP, _VP]),

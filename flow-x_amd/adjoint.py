@@ -78,31 +78,40 @@ class DiffSolver:
     def stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
-    def forward_raw(self, x, v, gravity, translation):
+    def new_decisions(self):
+        """A substep record (branches + neighbour lists) for forward_raw mode 1."""
+        nbytes = _lib.lib().fx_adjoint_record_bytes(ctypes.byref(self.params))
+        return torch.empty(int(nbytes) // 4, dtype=torch.int32, device=self.device)
+
+    def forward_raw(self, x, v, gravity, translation, decisions=None, mode=0):
+        """mode: 0 decide, 1 decide and record into `decisions`, 2 replay them."""
         x_out = torch.empty_like(x)
         v_out = torch.empty_like(v)
         tr = (ctypes.c_double * 3)(*[float(t) for t in translation])
         _lib.check(_lib.lib().fx_adjoint_forward(
             ctypes.byref(self.params), self.precision, ctypes.c_double(float(gravity)), tr,
             ctypes.c_double(self.list_skin), _p(self.collider.grid if self.collider else None),
+            _p(decisions), int(mode if decisions is not None else 0),
             _p(x), _p(v), _p(x_out), _p(v_out), _p(self.work), ctypes.c_size_t(self.work.numel()),
             self.stream()), "fx_adjoint_forward")
         return x_out, v_out
 
-    def backward_raw(self, x, v, gravity, translation, gx_out, gv_out):
+    def backward_raw(self, x, v, gravity, translation, gx_out, gv_out, decisions=None):
         gx = torch.zeros_like(x)
         gv = torch.zeros_like(v)
         self.grad_params.zero_()
         tr = (ctypes.c_double * 3)(*[float(t) for t in translation])
         _lib.check(_lib.lib().fx_adjoint_backward(
             ctypes.byref(self.params), self.precision, ctypes.c_double(float(gravity)), tr,
-            ctypes.c_double(self.list_skin), _p(self.collider.grid if self.collider else None),
+            ctypes.c_double(self.list_skin), _p(self.collider.grid if self.collider else None), _p(decisions),
             _p(x), _p(v), _p(gx_out.contiguous()), _p(gv_out.contiguous()), _p(gx), _p(gv),
             _p(self.grad_params), _p(self.work), ctypes.c_size_t(self.work.numel()), self.stream()),
             "fx_adjoint_backward")
         return gx, gv, self.grad_params.clone()
 
     def overflowed(self):
+        """Whether a neighbour list exceeded its capacity since the last call
+        (the device flag is cleared by reading it)."""
         flag = ctypes.c_int32(0)
         _lib.check(_lib.lib().fx_adjoint_overflow(ctypes.byref(self.params), self.precision, _p(self.work),
                                                   ctypes.byref(flag), self.stream()), "fx_adjoint_overflow")
@@ -126,15 +135,16 @@ class DiffSolver:
 
 
 class Substep(torch.autograd.Function):
-    """(x, v, gravity, translation) -> (x', v'); x, v: N x 4 tensors."""
+    """(x, v, gravity, translation) -> (x', v'); x, v: N x 4 tensors.
+    decisions/mode: see DiffSolver.forward_raw; backward replays them."""
 
     @staticmethod
-    def forward(ctx, x, v, gravity, translation, solver):
+    def forward(ctx, x, v, gravity, translation, solver, decisions=None, mode=0):
         g = float(gravity)
         tr = [float(t) for t in translation]
-        x_out, v_out = solver.forward_raw(x.contiguous(), v.contiguous(), g, tr)
+        x_out, v_out = solver.forward_raw(x.contiguous(), v.contiguous(), g, tr, decisions, mode)
         ctx.save_for_backward(x, v)  # the substep's checkpoint
-        ctx.solver, ctx.g, ctx.tr = solver, g, tr
+        ctx.solver, ctx.g, ctx.tr, ctx.dec = solver, g, tr, decisions
         ctx.gdtype = gravity.dtype, translation.dtype
         return x_out, v_out
 
@@ -145,8 +155,8 @@ class Substep(torch.autograd.Function):
             gx_out = torch.zeros_like(x)
         if gv_out is None:
             gv_out = torch.zeros_like(v)
-        gx, gv, gp = ctx.solver.backward_raw(x, v, ctx.g, ctx.tr, gx_out, gv_out)
-        return gx, gv, gp[0].to(ctx.gdtype[0]), gp[1:4].to(ctx.gdtype[1]), None
+        gx, gv, gp = ctx.solver.backward_raw(x, v, ctx.g, ctx.tr, gx_out, gv_out, ctx.dec)
+        return gx, gv, gp[0].to(ctx.gdtype[0]), gp[1:4].to(ctx.gdtype[1]), None, None, None
 
 
 class Splat(torch.autograd.Function):
@@ -162,10 +172,24 @@ class Splat(torch.autograd.Function):
         return ctx.solver.splat_backward_raw(x, gfield), None
 
 
-def rollout(solver, x0, v0, gravity, translation, substeps):
+def rollout(solver, x0, v0, gravity, translation, substeps, decisions=None, mode=0, check_overflow=True):
+    """`substeps` differentiable substeps.  With decisions = [] and mode 1 the
+    per-substep records (branches and neighbour lists) are appended to the
+    list; mode 2 replays a list recorded earlier.  Raises if a neighbour list
+    overflowed its capacity, since the gradient of a truncated list is not
+    that of the substep."""
     x, v = x0, v0
-    for _ in range(substeps):
-        x, v = Substep.apply(x, v, gravity, translation, solver)
+    for k in range(substeps):
+        dec = None
+        if decisions is not None:
+            if mode == 1:
+                dec = solver.new_decisions()
+                decisions.append(dec)
+            elif mode == 2:
+                dec = decisions[k]
+        x, v = Substep.apply(x, v, gravity, translation, solver, dec, mode)
+    if check_overflow and solver.overflowed():
+        raise RuntimeError("adjoint neighbour lists overflowed their capacity (local density too high)")
     return x, v
 
 

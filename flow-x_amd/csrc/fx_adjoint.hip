@@ -42,7 +42,7 @@ __device__ __forceinline__ V3<T> operator*(T s, V3<T> a) { return v3(s * a.x, s 
 template <class T>
 __device__ __forceinline__ T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
-constexpr int kMaxNbr = 192;
+constexpr int kMaxNbr = 192;  // list capacity: kMaxNbr pairs per particle on average
 
 template <class T>
 struct AdjC {
@@ -54,11 +54,21 @@ struct AdjC {
   const uint8_t* occ;
   int iters;
   T list_r2;
+  // recorded discrete decisions, 2 ints per particle: flags (bit 0 pushed
+  // out of the collider, bit 1 velocity reflected, bits 2-7 wall clamp per
+  // axis (1 low, 2 high), bit 8 speed clamped) and the chosen free voxel
+  int32_t* dec;
+  int dec_mode;  // 0 decide, 1 decide and record, 2 replay the record
 };
 
+enum { DEC_NONE = 0, DEC_RECORD = 1, DEC_REPLAY = 2 };
+
 template <class T>
-static AdjC<T> make_adjc(const FxParams& p, double gravity, const double tr[3], const uint8_t* col, double skin) {
+static AdjC<T> make_adjc(const FxParams& p, double gravity, const double tr[3], const uint8_t* col, double skin,
+                         int32_t* dec = nullptr, int dec_mode = DEC_NONE) {
   AdjC<T> c;
+  c.dec = dec;
+  c.dec_mode = dec ? dec_mode : DEC_NONE;
   const double h = p.h, pi = 3.14159265358979323846;
   c.h = (T)h;
   c.h2 = (T)(h * h);
@@ -146,11 +156,15 @@ __global__ void k_adj_to_f4(int n, const V3<T>* __restrict__ p, float4* __restri
 }
 
 // Each particle's neighbours within the list radius (27 cells of size h
-// around its cell), sorted by particle index; overflow is flagged.
-template <class T>
+// around its cell), in CSR form: a count pass, an exclusive scan into noff
+// (n + 1 entries), then a fill pass that sorts each list by particle index.
+// The pair total is capped at n * kMaxNbr; beyond it the offsets are clamped
+// and the overflow flag is raised.
+template <class T, bool FILL>
 __global__ void k_adj_lists(int n, Grid g, AdjC<T> c, const V3<T>* __restrict__ p, const float4* __restrict__ pf,
                             const int32_t* __restrict__ off, const int32_t* __restrict__ sidx,
-                            int32_t* __restrict__ nbr, int32_t* __restrict__ ncnt, int32_t* __restrict__ overflow) {
+                            int32_t* __restrict__ count, const int32_t* __restrict__ noff,
+                            int32_t* __restrict__ nbr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const V3<T> pi = p[i];
@@ -158,7 +172,12 @@ __global__ void k_adj_lists(int n, Grid g, AdjC<T> c, const V3<T>* __restrict__ 
   int bx, by, bz;
   cell_coord(g, qi.x, qi.y, qi.z, bx, by, bz);
   int m = 0;
-  int32_t* L = nbr + (size_t)i * kMaxNbr;
+  int32_t* L = nullptr;
+  int room = 0;
+  if (FILL) {
+    L = nbr + noff[i];
+    room = noff[i + 1] - noff[i];
+  }
   for (int dz = -1; dz <= 1; ++dz)
     for (int dy = -1; dy <= 1; ++dy)
       for (int dx = -1; dx <= 1; ++dx) {
@@ -170,15 +189,16 @@ __global__ void k_adj_lists(int n, Grid g, AdjC<T> c, const V3<T>* __restrict__ 
           if (j == i) continue;
           const V3<T> d = pi - p[j];
           if (dot(d, d) < c.list_r2) {
-            if (m < kMaxNbr)
-              L[m] = j;
-            else
-              atomicOr(overflow, 1);
+            if (FILL && m < room) L[m] = j;
             ++m;
           }
         }
       }
-  m = min(m, kMaxNbr);
+  if (!FILL) {
+    count[i] = m;
+    return;
+  }
+  m = min(m, room);
   for (int a = 1; a < m; ++a) {
     const int v = L[a];
     int b = a - 1;
@@ -188,7 +208,17 @@ __global__ void k_adj_lists(int n, Grid g, AdjC<T> c, const V3<T>* __restrict__ 
     }
     L[b + 1] = v;
   }
-  ncnt[i] = m;
+}
+
+// Clamp the list offsets to the capacity; flag an overflow.
+__global__ void k_adj_cap(int n, long long cap, int32_t* __restrict__ noff, int32_t* __restrict__ overflow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const int v = noff[i];
+  if (v > cap) {
+    noff[i] = (int32_t)cap;
+    if (i == n) atomicOr(overflow, 1);
+  }
 }
 
 // ---- forward passes ---------------------------------------------------------
@@ -204,7 +234,12 @@ __global__ void k_adj_predict(int n, AdjC<T> c, const V3<T>* __restrict__ x, con
   u.z = u.z + c.g * c.dt;
   const T vmax = T(0.4) * c.h / (c.dt > T(1e-6) ? c.dt : T(1e-6));
   const T speed = sqrt(dot(u, u));
-  if (speed > vmax) u = (vmax / speed) * u;
+  const bool clamp = c.dec_mode == DEC_REPLAY ? ((c.dec[2 * i] >> 8) & 1) != 0 : speed > vmax;
+  if (c.dec_mode == DEC_RECORD) {
+    c.dec[2 * i] = clamp ? 256 : 0;
+    c.dec[2 * i + 1] = -1;
+  }
+  if (clamp) u = (vmax / speed) * u;
   v1[i] = u;
   p0[i] = x[i] + c.dt * u;
 }
@@ -213,7 +248,7 @@ __global__ void k_adj_predict(int n, AdjC<T> c, const V3<T>* __restrict__ x, con
 // lambda and the gradient sum G for the backward pass.
 template <class T>
 __global__ void k_adj_lambda(int n, AdjC<T> c, const V3<T>* __restrict__ p, const int32_t* __restrict__ nbr,
-                             const int32_t* __restrict__ ncnt, T* __restrict__ rho, T* __restrict__ lam,
+                             const int32_t* __restrict__ noff, T* __restrict__ rho, T* __restrict__ lam,
                              T* __restrict__ Dv, V3<T>* __restrict__ G) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -221,8 +256,8 @@ __global__ void k_adj_lambda(int n, AdjC<T> c, const V3<T>* __restrict__ p, cons
   T dens = c.poly6 * c.h2 * c.h2 * c.h2;  // self term W(0)
   V3<T> gs = v3(T(0), T(0), T(0));
   T gn2 = T(0);
-  const int32_t* L = nbr + (size_t)i * kMaxNbr;
-  for (int k = 0; k < ncnt[i]; ++k) {
+  const int32_t* L = nbr + noff[i];
+  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
     const V3<T> d = pi - p[L[k]];
     const T r2 = dot(d, d);
     if (r2 < c.h2) {
@@ -250,15 +285,15 @@ __global__ void k_adj_lambda(int n, AdjC<T> c, const V3<T>* __restrict__ p, cons
 // shaders/sph_delta.glsl:22-85 + sph_apply_delta.glsl (p_next = p + dp)
 template <class T>
 __global__ void k_adj_delta(int n, AdjC<T> c, const V3<T>* __restrict__ p, const T* __restrict__ lam,
-                            const int32_t* __restrict__ nbr, const int32_t* __restrict__ ncnt,
+                            const int32_t* __restrict__ nbr, const int32_t* __restrict__ noff,
                             V3<T>* __restrict__ pn) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const V3<T> pi = p[i];
   const T li = lam[i];
   V3<T> acc = v3(T(0), T(0), T(0));
-  const int32_t* L = nbr + (size_t)i * kMaxNbr;
-  for (int k = 0; k < ncnt[i]; ++k) {
+  const int32_t* L = nbr + noff[i];
+  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
     const int j = L[k];
     const V3<T> d = pi - p[j];
     const T r2 = dot(d, d);
@@ -286,10 +321,46 @@ __device__ __forceinline__ bool adj_occupied(const AdjC<T>& c, int x, int y, int
 }
 
 template <class T>
-__device__ void adj_finalize(const AdjC<T>& c, V3<T>& p, V3<T>& v, FinalRec<T>& rec) {
+__device__ void adj_finalize(const AdjC<T>& c, int i, V3<T>& p, V3<T>& v, FinalRec<T>& rec) {
   rec.inside = 0;
   rec.damped = 0;
   rec.clamped[0] = rec.clamped[1] = rec.clamped[2] = 0;
+  if (c.dec_mode == DEC_REPLAY) {
+    const int flags = c.dec[2 * i];
+    if ((flags & 1) && c.occ) {
+      const int idx = c.dec[2 * i + 1];
+      const int x = idx % c.kx, y = (idx / c.kx) % c.ky, z = idx / (c.kx * c.ky);
+      const V3<T> ctr = v3(c.lo[0] + ((T)x + T(0.5)) * c.voxel + c.tr[0], c.lo[1] + ((T)y + T(0.5)) * c.voxel + c.tr[1],
+                           c.lo[2] + ((T)z + T(0.5)) * c.voxel + c.tr[2]);
+      const V3<T> e = ctr - p;
+      const T dist = sqrt(dot(e, e));
+      const V3<T> u = dist > T(1e-6) ? (T(1) / dist) * e : v3(T(0), T(0), T(1));
+      rec.inside = 1;
+      rec.centre = ctr;
+      p = p + (dist + c.radius) * u;
+      if (flags & 2) {
+        const T vn = dot(v, u);
+        v = v - (vn * (T(1) + c.damping)) * u;
+        rec.damped = 1;
+      }
+    }
+    T* pp[3] = {&p.x, &p.y, &p.z};
+    T* vv[3] = {&v.x, &v.y, &v.z};
+    for (int a = 0; a < 3; ++a) {
+      const int code = (flags >> (2 + 2 * a)) & 3;
+      if (code == 1) {
+        *pp[a] = c.lo[a] + c.radius;
+        *vv[a] = -*vv[a] * c.damping;
+        rec.clamped[a] = -1;
+      } else if (code == 2) {
+        *pp[a] = c.hi[a] - c.radius;
+        *vv[a] = -*vv[a] * c.damping;
+        rec.clamped[a] = 1;
+      }
+    }
+    return;
+  }
+  int vox = -1;
   if (c.occ) {
     const int cx = (int)floor((p.x - c.tr[0] - c.lo[0]) / c.voxel);
     const int cy = (int)floor((p.y - c.tr[1] - c.lo[1]) / c.voxel);
@@ -313,6 +384,7 @@ __device__ void adj_finalize(const AdjC<T>& c, V3<T>& p, V3<T>& v, FinalRec<T>& 
               best = d2;
               nf = ctr;
               found = true;
+              vox = (z * c.ky + y) * c.kx + x;
             }
           }
       if (found) {
@@ -343,6 +415,12 @@ __device__ void adj_finalize(const AdjC<T>& c, V3<T>& p, V3<T>& v, FinalRec<T>& 
       rec.clamped[a] = 1;
     }
   }
+  if (c.dec_mode == DEC_RECORD) {
+    int flags = (c.dec[2 * i] & 256) | (rec.inside ? 1 : 0) | (rec.damped ? 2 : 0);
+    for (int a = 0; a < 3; ++a) flags |= (rec.clamped[a] == -1 ? 1 : rec.clamped[a] == 1 ? 2 : 0) << (2 + 2 * a);
+    c.dec[2 * i] = flags;
+    c.dec[2 * i + 1] = rec.inside ? vox : -1;
+  }
 }
 
 // velocity + XSPH (density from the last lambda pass) + finalize
@@ -358,14 +436,14 @@ __global__ void k_adj_velocity(int n, AdjC<T> c, const V3<T>* __restrict__ pI, c
 template <class T>
 __global__ void k_adj_xsph_final(int n, AdjC<T> c, const V3<T>* __restrict__ pI, const V3<T>* __restrict__ v2,
                                  const T* __restrict__ rho, const int32_t* __restrict__ nbr,
-                                 const int32_t* __restrict__ ncnt, V3<T>* __restrict__ x_out,
+                                 const int32_t* __restrict__ noff, V3<T>* __restrict__ x_out,
                                  V3<T>* __restrict__ v_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const V3<T> pi = pI[i], vi = v2[i];
   V3<T> acc = v3(T(0), T(0), T(0));
-  const int32_t* L = nbr + (size_t)i * kMaxNbr;
-  for (int k = 0; k < ncnt[i]; ++k) {
+  const int32_t* L = nbr + noff[i];
+  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
     const int j = L[k];
     const V3<T> d = pi - pI[j];
     const T r2 = dot(d, d);
@@ -376,7 +454,7 @@ __global__ void k_adj_xsph_final(int n, AdjC<T> c, const V3<T>* __restrict__ pI,
   }
   V3<T> p = pi, v = vi + c.visc * acc;
   FinalRec<T> rec;
-  adj_finalize(c, p, v, rec);
+  adj_finalize(c, i, p, v, rec);
   x_out[i] = p;
   v_out[i] = v;
 }
@@ -388,7 +466,7 @@ __global__ void k_adj_xsph_final(int n, AdjC<T> c, const V3<T>* __restrict__ pI,
 template <class T>
 __global__ void k_adj_final_bwd(int n, AdjC<T> c, const V3<T>* __restrict__ pI, const V3<T>* __restrict__ v2,
                                 const T* __restrict__ rho, const int32_t* __restrict__ nbr,
-                                const int32_t* __restrict__ ncnt, const V3<T>* __restrict__ gxo,
+                                const int32_t* __restrict__ noff, const V3<T>* __restrict__ gxo,
                                 const V3<T>* __restrict__ gvo, V3<T>* __restrict__ gpI, V3<T>* __restrict__ gv3,
                                 V3<T>* __restrict__ gt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -396,8 +474,8 @@ __global__ void k_adj_final_bwd(int n, AdjC<T> c, const V3<T>* __restrict__ pI, 
   // recompute the pre-finalize state
   const V3<T> pi = pI[i], vi = v2[i];
   V3<T> acc = v3(T(0), T(0), T(0));
-  const int32_t* L = nbr + (size_t)i * kMaxNbr;
-  for (int k = 0; k < ncnt[i]; ++k) {
+  const int32_t* L = nbr + noff[i];
+  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
     const int j = L[k];
     const V3<T> d = pi - pI[j];
     const T r2 = dot(d, d);
@@ -409,7 +487,7 @@ __global__ void k_adj_final_bwd(int n, AdjC<T> c, const V3<T>* __restrict__ pI, 
   const V3<T> v3in = vi + c.visc * acc;
   V3<T> p = pi, v = v3in;
   FinalRec<T> rec;
-  adj_finalize(c, p, v, rec);
+  adj_finalize(c, i, p, v, rec);
   V3<T> gp = gxo[i], gv = gvo[i];
   // domain clamp (applied last): clamped position components have no
   // dependence on the input; clamped velocity components scale by -damping
@@ -460,7 +538,7 @@ __global__ void k_adj_final_bwd(int n, AdjC<T> c, const V3<T>* __restrict__ pI, 
 template <class T>
 __global__ void k_adj_xsph_bwd(int n, AdjC<T> c, const V3<T>* __restrict__ pI, const V3<T>* __restrict__ v2,
                                const T* __restrict__ rho, const int32_t* __restrict__ nbr,
-                               const int32_t* __restrict__ ncnt, const V3<T>* __restrict__ gv3,
+                               const int32_t* __restrict__ noff, const V3<T>* __restrict__ gv3,
                                V3<T>* __restrict__ gpI, V3<T>* __restrict__ gv2, T* __restrict__ grho) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -468,8 +546,8 @@ __global__ void k_adj_xsph_bwd(int n, AdjC<T> c, const V3<T>* __restrict__ pI, c
   const T ri = rho[i];
   V3<T> gp = v3(T(0), T(0), T(0)), gv = gi;
   T gr = T(0);
-  const int32_t* L = nbr + (size_t)i * kMaxNbr;
-  for (int k = 0; k < ncnt[i]; ++k) {
+  const int32_t* L = nbr + noff[i];
+  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
     const int j = L[k];
     const V3<T> d = pi - pI[j];
     const T r2 = dot(d, d);
@@ -507,15 +585,15 @@ __global__ void k_adj_velocity_bwd(int n, AdjC<T> c, const V3<T>* __restrict__ g
 template <class T>
 __global__ void k_adj_iter_bwd1(int n, AdjC<T> c, const V3<T>* __restrict__ p, const T* __restrict__ rho,
                                 const T* __restrict__ lam, const T* __restrict__ Dv, const V3<T>* __restrict__ G,
-                                const int32_t* __restrict__ nbr, const int32_t* __restrict__ ncnt,
+                                const int32_t* __restrict__ nbr, const int32_t* __restrict__ noff,
                                 const V3<T>* __restrict__ a, const T* __restrict__ grho_extra,
                                 T* __restrict__ grho, V3<T>* __restrict__ gG, T* __restrict__ gS) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const V3<T> pi = p[i], ai = a[i];
   T gl = T(0);
-  const int32_t* L = nbr + (size_t)i * kMaxNbr;
-  for (int k = 0; k < ncnt[i]; ++k) {
+  const int32_t* L = nbr + noff[i];
+  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
     const int j = L[k];
     const V3<T> d = pi - p[j];
     V3<T> ph;
@@ -536,7 +614,7 @@ __global__ void k_adj_iter_bwd1(int n, AdjC<T> c, const V3<T>* __restrict__ p, c
 // step 2: adjoint of p_it.
 template <class T>
 __global__ void k_adj_iter_bwd2(int n, AdjC<T> c, const V3<T>* __restrict__ p, const T* __restrict__ lam,
-                                const int32_t* __restrict__ nbr, const int32_t* __restrict__ ncnt,
+                                const int32_t* __restrict__ nbr, const int32_t* __restrict__ noff,
                                 const V3<T>* __restrict__ a, const T* __restrict__ grho,
                                 const V3<T>* __restrict__ gG, const T* __restrict__ gS,
                                 V3<T>* __restrict__ gp_out) {
@@ -545,8 +623,8 @@ __global__ void k_adj_iter_bwd2(int n, AdjC<T> c, const V3<T>* __restrict__ p, c
   const V3<T> pi = p[i], ai = a[i], gGi = gG[i];
   const T li = lam[i], gSi = gS[i], gri = grho[i];
   V3<T> acc = ai;
-  const int32_t* L = nbr + (size_t)i * kMaxNbr;
-  for (int k = 0; k < ncnt[i]; ++k) {
+  const int32_t* L = nbr + noff[i];
+  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
     const int j = L[k];
     const V3<T> d = pi - p[j];
     const T r2 = dot(d, d);
@@ -578,7 +656,8 @@ __global__ void k_adj_predict_bwd(int n, AdjC<T> c, const V3<T>* __restrict__ v,
   const T speed = sqrt(dot(u, u));
   const V3<T> g0 = gp0[i];
   V3<T> g1 = c.dt * g0;  // adjoint of the clamped velocity
-  if (speed > vmax) {
+  const bool clamp = c.dec_mode == DEC_REPLAY ? ((c.dec[2 * i] >> 8) & 1) != 0 : speed > vmax;
+  if (clamp) {
     // u' = vmax u / |u|: J = (vmax/|u|)(I - uhat uhat^T)
     const V3<T> uh = (T(1) / speed) * u;
     g1 = (vmax / speed) * (g1 - dot(uh, g1) * uh);
@@ -678,15 +757,14 @@ __global__ void k_adj_reduce_final(int nb, const T* __restrict__ in, double* __r
   *out += s;
 }
 
-__global__ void k_adj_zero(int32_t* a, int n, int32_t* flag) {
+__global__ void k_adj_zero(int32_t* a, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
-  if (i == 0) *flag = 0;
 }
 
 // ---- host side --------------------------------------------------------------
 struct AdjWs {
-  size_t cnt, off, rank, key, sidx, part, pf, nbr, ncnt, flag;
+  size_t cnt, off, rank, key, sidx, part, pf, nbr, noff, lcnt, flag;
   size_t v1, P, rho, lam, D, G, v2, a, b, grho, gG, gS, gv2, gv3, gt, gg, red;
   size_t total;
 };
@@ -701,10 +779,11 @@ static AdjWs adj_layout(const FxParams& p, int prec) {
   w.rank = o; o += al(n * 4 + 4);
   w.key = o;  o += al(n * 4 + 4);
   w.sidx = o; o += al(n * 4 + 4);
-  w.part = o; o += al(scan_partials((long long)c + 1) * 4 + 4);
+  w.part = o; o += al(scan_partials((long long)std::max(c, n) + 1) * 4 + 4);
   w.pf = o;   o += al(n * 16 + 16);
   w.nbr = o;  o += al(n * kMaxNbr * 4 + 4);
-  w.ncnt = o; o += al(n * 4 + 4);
+  w.noff = o; o += al(n * 4 + 8);
+  w.lcnt = o; o += al(n * 4 + 4);
   w.flag = o; o += al(64);
   w.v1 = o;   o += al(n * 4 * T + 64);
   w.P = o;    o += al((I + 1) * n * 4 * T + 64);
@@ -736,12 +815,31 @@ static int adj_validate(const FxParams* p, int prec, void* work, size_t work_byt
       (long long)p->cells[0] * p->cells[1] * p->cells[2] != p->n_cells)
     return fail(FX_EINVAL, "adjoint: bad particle count or cell dims");
   if (p->iterations < 1) return fail(FX_EINVAL, "adjoint: iterations must be >= 1");
+  if ((long long)p->n * kMaxNbr >= (1LL << 31))
+    return fail(FX_ERANGE, "adjoint: %d particles exceed the 32-bit neighbour-list capacity", p->n);
   if (!work || work_bytes < adj_layout(*p, prec).total)
     return fail(FX_ENOMEM, "adjoint workspace too small: %zu < %zu", work_bytes, adj_layout(*p, prec).total);
   return FX_OK;
 }
 
 #define ALAUNCH(kernel, n, st, ...) FX_LAUNCH(kernel, dim3(grid_for(n)), dim3(kBlock), st, __VA_ARGS__)
+
+// A substep's record: [0, 2n) branch decisions (flags, voxel), [2n, 3n + 1)
+// list offsets, [3n + 1, 3n + 1 + n * kMaxNbr) neighbour lists.  Recording
+// keeps the lists there; replaying reuses them instead of rebuilding from
+// the new state.
+static size_t adj_record_ints(int n) { return n > 0 ? (size_t)n * (3 + kMaxNbr) + 1 : 0; }
+
+template <class T>
+static void adj_list_ptrs(const AdjC<T>& c, int n, char* b, const AdjWs& w, int32_t*& nbr, int32_t*& noff) {
+  if (c.dec_mode != DEC_NONE) {
+    noff = c.dec + 2 * (size_t)n;
+    nbr = c.dec + 3 * (size_t)n + 1;
+  } else {
+    noff = (int32_t*)(b + w.noff);
+    nbr = (int32_t*)(b + w.nbr);
+  }
+}
 
 // Forward substep; keeps every iterate in the workspace.
 template <class T>
@@ -758,30 +856,37 @@ static int adj_forward(const FxParams& p, const AdjC<T>& c, const V3<T>* x, cons
   V3<T>* v2 = (V3<T>*)(b + w.v2);
   int32_t *cnt = (int32_t*)(b + w.cnt), *off = (int32_t*)(b + w.off), *rank = (int32_t*)(b + w.rank),
           *key = (int32_t*)(b + w.key), *sidx = (int32_t*)(b + w.sidx), *part = (int32_t*)(b + w.part),
-          *nbr = (int32_t*)(b + w.nbr), *ncnt = (int32_t*)(b + w.ncnt), *flag = (int32_t*)(b + w.flag);
+          *flag = (int32_t*)(b + w.flag), *lcnt = (int32_t*)(b + w.lcnt), *nbr, *noff;
+  adj_list_ptrs(c, n, b, w, nbr, noff);
   float4* pf = (float4*)(b + w.pf);
   const Grid g = make_grid(p);
   ALAUNCH(k_adj_predict<T>, n, st, n, c, x, v, P, v1);
-  ALAUNCH(k_adj_to_f4<T>, n, st, n, (const V3<T>*)P, pf);
-  FX_LAUNCH(k_adj_zero, dim3(grid_for(nc + 1)), dim3(kBlock), st, cnt, nc + 1, flag);
-  ALAUNCH(k_adj_key, n, st, n, g, (const float4*)pf, key, cnt, rank);
-  FX_TRY(exclusive_scan_i32(cnt, off, nc, part, st));
-  ALAUNCH(k_adj_scatter, n, st, n, (const int32_t*)key, (const int32_t*)rank, (const int32_t*)off, sidx);
-  ALAUNCH(k_adj_lists<T>, n, st, n, g, c, (const V3<T>*)P, (const float4*)pf, (const int32_t*)off,
-          (const int32_t*)sidx, nbr, ncnt, flag);
+  if (c.dec_mode != DEC_REPLAY) {
+    ALAUNCH(k_adj_to_f4<T>, n, st, n, (const V3<T>*)P, pf);
+    FX_LAUNCH(k_adj_zero, dim3(grid_for(nc + 1)), dim3(kBlock), st, cnt, nc + 1);
+    ALAUNCH(k_adj_key, n, st, n, g, (const float4*)pf, key, cnt, rank);
+    FX_TRY(exclusive_scan_i32(cnt, off, nc, part, st));
+    ALAUNCH(k_adj_scatter, n, st, n, (const int32_t*)key, (const int32_t*)rank, (const int32_t*)off, sidx);
+    ALAUNCH((k_adj_lists<T, false>), n, st, n, g, c, (const V3<T>*)P, (const float4*)pf, (const int32_t*)off,
+            (const int32_t*)sidx, lcnt, (const int32_t*)nullptr, (int32_t*)nullptr);
+    FX_TRY(exclusive_scan_i32(lcnt, noff, n, part, st));
+    FX_LAUNCH(k_adj_cap, dim3(grid_for(n + 1)), dim3(kBlock), st, n, (long long)n * kMaxNbr, noff, flag);
+    ALAUNCH((k_adj_lists<T, true>), n, st, n, g, c, (const V3<T>*)P, (const float4*)pf, (const int32_t*)off,
+            (const int32_t*)sidx, (int32_t*)nullptr, (const int32_t*)noff, nbr);
+  }
   for (int it = 0; it < I; ++it) {
     V3<T>* Pi = P + (size_t)it * n;
-    ALAUNCH(k_adj_lambda<T>, n, st, n, c, (const V3<T>*)Pi, (const int32_t*)nbr, (const int32_t*)ncnt,
+    ALAUNCH(k_adj_lambda<T>, n, st, n, c, (const V3<T>*)Pi, (const int32_t*)nbr, (const int32_t*)noff,
             rho + (size_t)it * n, lam + (size_t)it * n, D + (size_t)it * n, G + (size_t)it * n);
     ALAUNCH(k_adj_delta<T>, n, st, n, c, (const V3<T>*)Pi, (const T*)(lam + (size_t)it * n), (const int32_t*)nbr,
-            (const int32_t*)ncnt, P + (size_t)(it + 1) * n);
+            (const int32_t*)noff, P + (size_t)(it + 1) * n);
   }
   const V3<T>* PI = P + (size_t)I * n;
   const T* rho_last = rho + (size_t)(I - 1) * n;
   ALAUNCH(k_adj_velocity<T>, n, st, n, c, PI, x, v2);
   if (x_out && v_out)
     ALAUNCH(k_adj_xsph_final<T>, n, st, n, c, PI, (const V3<T>*)v2, rho_last, (const int32_t*)nbr,
-            (const int32_t*)ncnt, x_out, v_out);
+            (const int32_t*)noff, x_out, v_out);
   return check_launch("adjoint forward");
 }
 
@@ -818,15 +923,17 @@ static int adj_backward(const FxParams& p, const AdjC<T>& c, const V3<T>* x, con
   V3<T>* gt = (V3<T>*)(b + w.gt);
   T* gg = (T*)(b + w.gg);
   T* grho_x = (T*)(b + w.gS);  // reuse: XSPH's density adjoint, consumed before gS is written
-  const int32_t* nbr = (const int32_t*)(b + w.nbr);
-  const int32_t* ncnt = (const int32_t*)(b + w.ncnt);
+  int32_t *nbr_w, *noff_w;
+  adj_list_ptrs(c, n, b, w, nbr_w, noff_w);
+  const int32_t* nbr = nbr_w;
+  const int32_t* noff = noff_w;
   const V3<T>* PI = P + (size_t)I * n;
   const T* rho_last = rho + (size_t)(I - 1) * n;
   // finalize, XSPH, velocity
-  ALAUNCH(k_adj_final_bwd<T>, n, st, n, c, PI, (const V3<T>*)v2, rho_last, nbr, ncnt, gxo, gvo, A, gv3, gt);
+  ALAUNCH(k_adj_final_bwd<T>, n, st, n, c, PI, (const V3<T>*)v2, rho_last, nbr, noff, gxo, gvo, A, gv3, gt);
   for (int a = 0; a < 3; ++a) FX_TRY(adj_reduce<T>((const T*)gt, n, 4, a, b, w, gparams + 1 + a, st));
   T* grho_tmp = (T*)(b + w.grho);
-  ALAUNCH(k_adj_xsph_bwd<T>, n, st, n, c, PI, (const V3<T>*)v2, rho_last, nbr, ncnt, (const V3<T>*)gv3, A, gv2,
+  ALAUNCH(k_adj_xsph_bwd<T>, n, st, n, c, PI, (const V3<T>*)v2, rho_last, nbr, noff, (const V3<T>*)gv3, A, gv2,
           grho_tmp);
   // keep XSPH's density adjoint apart: copy into the gS slot
   FX_HIP_CHECK(hipMemcpyAsync(grho_x, grho_tmp, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, st));
@@ -838,9 +945,9 @@ static int adj_backward(const FxParams& p, const AdjC<T>& c, const V3<T>* x, con
     const V3<T>* Pi = P + (size_t)it * n;
     const T* li = lam + (size_t)it * n;
     ALAUNCH(k_adj_iter_bwd1<T>, n, st, n, c, Pi, (const T*)(rho + (size_t)it * n), li, (const T*)(D + (size_t)it * n),
-            (const V3<T>*)(G + (size_t)it * n), nbr, ncnt, (const V3<T>*)A,
+            (const V3<T>*)(G + (size_t)it * n), nbr, noff, (const V3<T>*)A,
             (const T*)(it == I - 1 ? grho_extra : nullptr), grho, gG, gS);
-    ALAUNCH(k_adj_iter_bwd2<T>, n, st, n, c, Pi, li, nbr, ncnt, (const V3<T>*)A, (const T*)grho, (const V3<T>*)gG,
+    ALAUNCH(k_adj_iter_bwd2<T>, n, st, n, c, Pi, li, nbr, noff, (const V3<T>*)A, (const T*)grho, (const V3<T>*)gG,
             (const T*)gS, B);
     V3<T>* tmp = A;
     A = B;
@@ -857,29 +964,36 @@ using namespace fx;
 
 extern "C" {
 
+size_t fx_adjoint_record_bytes(const FxParams* p) { return p ? adj_record_ints(p->n) * 4 : 0; }
+
 size_t fx_adjoint_workspace_bytes(const FxParams* p, int precision) {
   if (!p || (precision != 4 && precision != 8)) return 0;
   return adj_layout(*p, precision).total;
 }
 
 int fx_adjoint_forward(const FxParams* p, int precision, double gravity, const double translation[3],
-                       double list_skin, const uint8_t* collider, const void* x, const void* v, void* x_out,
-                       void* v_out, void* work, size_t work_bytes, void* stream) {
+                       double list_skin, const uint8_t* collider, int32_t* decisions, int decision_mode,
+                       const void* x, const void* v, void* x_out, void* v_out, void* work, size_t work_bytes,
+                       void* stream) {
   FX_TRY(adj_validate(p, precision, work, work_bytes));
   if (p->n > 0 && (!x || !v || !x_out || !v_out)) return fail(FX_EINVAL, "adjoint forward: null state");
+  if (decision_mode < DEC_NONE || decision_mode > DEC_REPLAY) return fail(FX_EINVAL, "adjoint: bad decision mode");
   const AdjWs w = adj_layout(*p, precision);
   hipStream_t st = (hipStream_t)stream;
   if (precision == 8)
-    return adj_forward<double>(*p, make_adjc<double>(*p, gravity, translation, collider, list_skin),
+    return adj_forward<double>(*p, make_adjc<double>(*p, gravity, translation, collider, list_skin, decisions,
+                                                     decision_mode),
                                (const V3<double>*)x, (const V3<double>*)v, (V3<double>*)x_out, (V3<double>*)v_out,
                                (char*)work, w, st);
-  return adj_forward<float>(*p, make_adjc<float>(*p, gravity, translation, collider, list_skin),
+  return adj_forward<float>(*p, make_adjc<float>(*p, gravity, translation, collider, list_skin, decisions,
+                                                   decision_mode),
                             (const V3<float>*)x, (const V3<float>*)v, (V3<float>*)x_out, (V3<float>*)v_out,
                             (char*)work, w, st);
 }
 
 int fx_adjoint_backward(const FxParams* p, int precision, double gravity, const double translation[3],
-                        double list_skin, const uint8_t* collider, const void* x, const void* v,
+                        double list_skin, const uint8_t* collider, int32_t* decisions, const void* x,
+                        const void* v,
                         const void* gx_out, const void* gv_out, void* gx, void* gv, double* grad_params,
                         void* work, size_t work_bytes, void* stream) {
   FX_TRY(adj_validate(p, precision, work, work_bytes));
@@ -888,11 +1002,13 @@ int fx_adjoint_backward(const FxParams* p, int precision, double gravity, const 
   const AdjWs w = adj_layout(*p, precision);
   hipStream_t st = (hipStream_t)stream;
   if (precision == 8)
-    return adj_backward<double>(*p, make_adjc<double>(*p, gravity, translation, collider, list_skin),
+    return adj_backward<double>(*p, make_adjc<double>(*p, gravity, translation, collider, list_skin, decisions,
+                                                      DEC_REPLAY),
                                 (const V3<double>*)x, (const V3<double>*)v, (const V3<double>*)gx_out,
                                 (const V3<double>*)gv_out, (V3<double>*)gx, (V3<double>*)gv, grad_params,
                                 (char*)work, w, st);
-  return adj_backward<float>(*p, make_adjc<float>(*p, gravity, translation, collider, list_skin),
+  return adj_backward<float>(*p, make_adjc<float>(*p, gravity, translation, collider, list_skin, decisions,
+                                                    DEC_REPLAY),
                              (const V3<float>*)x, (const V3<float>*)v, (const V3<float>*)gx_out,
                              (const V3<float>*)gv_out, (V3<float>*)gx, (V3<float>*)gv, grad_params,
                              (char*)work, w, st);
@@ -901,7 +1017,9 @@ int fx_adjoint_backward(const FxParams* p, int precision, double gravity, const 
 int fx_adjoint_overflow(const FxParams* p, int precision, const void* work, int32_t* host_flag, void* stream) {
   if (!p || !work || !host_flag) return fail(FX_EINVAL, "fx_adjoint_overflow: null argument");
   const AdjWs w = adj_layout(*p, precision);
-  FX_HIP_CHECK(hipMemcpyAsync(host_flag, (const char*)work + w.flag, 4, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  int32_t* flag = (int32_t*)((char*)work + w.flag);
+  FX_HIP_CHECK(hipMemcpyAsync(host_flag, flag, 4, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  FX_LAUNCH(k_adj_zero, dim3(1), dim3(64), (hipStream_t)stream, flag, 1);
   FX_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
   return FX_OK;
 }
@@ -927,7 +1045,7 @@ int fx_adjoint_splat(const FxParams* p, const FxSurface* surf, int precision, co
   if (q.n_cells > nc) return fail(FX_EINVAL, "fx_adjoint_splat: kernel radius below the cell size");
   const Grid g = make_grid(q);
   const long long S = (long long)surf->dims[0] * surf->dims[1] * surf->dims[2];
-  FX_LAUNCH(k_adj_zero, dim3(grid_for(q.n_cells + 1)), dim3(kBlock), st, cnt, q.n_cells + 1, flag);
+  FX_LAUNCH(k_adj_zero, dim3(grid_for(q.n_cells + 1)), dim3(kBlock), st, cnt, q.n_cells + 1);
   const double pi = 3.14159265358979323846;
   const double coef = 315.0 / (64.0 * pi * std::pow((double)kr, 9.0));
   if (precision == 8) {

@@ -227,15 +227,27 @@ int fx_batch_substep(const FxScene* scenes, int count, float* pos, float* vel,
  * adjoints of x and v given those of the outputs; grad_params (device
  * double[4]) accumulates d/dgravity and d/dtranslation.  surface_tension and
  * scorr_k must be 0.  fx_adjoint_overflow reports whether a neighbour list
- * exceeded its 192 entries in the last forward. */
+ * exceeded its 192 entries in the last forward.
+ * decisions (device memory of fx_adjoint_record_bytes, or NULL): the
+ * substep's record - its discrete branches (collider push-out and the chosen
+ * voxel, velocity reflection, wall clamps, speed clamp) and its neighbour
+ * lists - written by a forward with decision_mode 1 and replayed with mode 2
+ * (mode 0: decide and build lists in the workspace, no record).  Replaying
+ * freezes the branches and the neighbour structure of a perturbed run, so
+ * finite differences see the same smooth piece the adjoint differentiates,
+ * and skips the list build; fx_adjoint_backward always replays when
+ * decisions is non-NULL. */
 size_t fx_adjoint_workspace_bytes(const FxParams* p, int precision);
+size_t fx_adjoint_record_bytes(const FxParams* p);
 int fx_adjoint_forward(const FxParams* p, int precision, double gravity, const double translation[3],
-                       double list_skin, const uint8_t* collider, const void* x, const void* v,
-                       void* x_out, void* v_out, void* work, size_t work_bytes, void* stream);
+                       double list_skin, const uint8_t* collider, int32_t* decisions,
+                       int decision_mode, const void* x, const void* v, void* x_out, void* v_out,
+                       void* work, size_t work_bytes, void* stream);
 int fx_adjoint_backward(const FxParams* p, int precision, double gravity, const double translation[3],
-                        double list_skin, const uint8_t* collider, const void* x, const void* v,
-                        const void* gx_out, const void* gv_out, void* gx, void* gv,
-                        double* grad_params, void* work, size_t work_bytes, void* stream);
+                        double list_skin, const uint8_t* collider, int32_t* decisions,
+                        const void* x, const void* v, const void* gx_out, const void* gv_out,
+                        void* gx, void* gv, double* grad_params, void* work, size_t work_bytes,
+                        void* stream);
 int fx_adjoint_overflow(const FxParams* p, int precision, const void* work, int32_t* host_flag,
                         void* stream);
 /* Splat of the density field (shaders/surface_splat.glsl arithmetic, sample

@@ -54,10 +54,10 @@ def test_adjoint_forward_matches_plain_substep(fx, with_collider):
         plain.substep(dt)
 
 
-def _loss(fx, d, x0, v0, g, tr, steps, target, com_t):
+def _loss(fx, d, x0, v0, g, tr, steps, target, com_t, decisions=None, mode=0):
     from flowx_amd.adjoint import loss_fn, rollout
 
-    x, v = rollout(d, x0, v0, g, tr, steps)
+    x, v = rollout(d, x0, v0, g, tr, steps, decisions, mode)
     return loss_fn(d, x, target, com_t)
 
 
@@ -80,7 +80,8 @@ def test_adjoint_matches_finite_differences_fp64(fx):
     com_t = torch.tensor([0.05, 0.0, 0.15], dtype=torch.float64, device="cuda")
     steps = 4
     v = v0.clone().requires_grad_(True)
-    L = _loss(fx, d, x0, v, g, tr, steps, target, com_t)
+    decisions = []
+    L = _loss(fx, d, x0, v, g, tr, steps, target, com_t, decisions, 1)
     L.backward()
     gv, gg, gt = v.grad.clone(), g.grad.item(), tr.grad.cpu().numpy()
 
@@ -88,7 +89,7 @@ def test_adjoint_matches_finite_differences_fp64(fx):
         with torch.no_grad():
             vv = v0 + (dv if dv is not None else 0)
             tt = tr.detach() + (torch.as_tensor(dtr, dtype=torch.float64, device="cuda") if dtr is not None else 0)
-            return _loss(fx, d, x0, vv, g.detach() + dg, tt, steps, target, com_t).item()
+            return _loss(fx, d, x0, vv, g.detach() + dg, tt, steps, target, com_t, decisions, 2).item()
 
     eps = 1e-6
     worst = 0.0
@@ -114,3 +115,90 @@ def test_adjoint_matches_finite_differences_fp64(fx):
         else:
             assert abs(gt[a]) < 1e-6
     assert worst < 1e-3
+
+
+def _cluster_scene(fx, lattice, cluster, seed=3):
+    """A lattice plus `cluster` particles packed into a sphere of radius
+    0.4 h: the cluster's neighbour lists are far longer than the average."""
+    import torch
+
+    from flowx_amd.adjoint import DiffSolver
+    from flowx_amd.config import converging_update
+
+    cfg = converging_update(fx.lattice_config((-0.5, -0.5, 0.0), (0.5, 0.5, 1.0), 0.1, lattice, iterations=4))
+    cfg.extra.update(colliders=[], surface_resolution=16, fps=24.0)
+    base = fx.seed_positions(cfg) if lattice[0] else np.zeros((0, 4), np.float32)
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(cluster, 3))
+    u *= (0.04 * rng.uniform(0, 1, (cluster, 1)) ** (1 / 3)) / np.linalg.norm(u, axis=1, keepdims=True)
+    extra = np.concatenate([u + np.array([0.0, 0.0, 0.3]), np.ones((cluster, 1))], 1).astype(np.float32)
+    pos = np.concatenate([base, extra])
+    cfg.particle_count = len(pos)
+    d = DiffSolver(cfg, dtype=torch.float64)
+    x0 = torch.as_tensor(pos, dtype=torch.float64).cuda()
+    x0[:, 3] = 0
+    return cfg, d, x0
+
+
+def test_adjoint_long_lists_in_dense_cluster(fx):
+    """Lists longer than the average capacity (192) in a dense cluster: the
+    adjoint still matches float64 finite differences, and replaying the
+    record reproduces the recorded substep exactly."""
+    import torch
+
+    from flowx_amd.adjoint import gaussian_target, loss_fn, rollout
+
+    cfg, d, x0 = _cluster_scene(fx, (20, 20, 8), 250)
+    n = len(x0)
+    rng = np.random.default_rng(4)
+    v0 = torch.as_tensor(rng.normal(0, 0.05, (n, 4)), dtype=torch.float64).cuda()
+    v0[:, 3] = 0
+    g = torch.tensor(-9.81, dtype=torch.float64, device="cuda", requires_grad=True)
+    tr = torch.zeros(3, dtype=torch.float64, device="cuda", requires_grad=True)
+    target = gaussian_target(d, (0.0, 0.0, 0.25), sigma=0.2)
+    com_t = torch.tensor([0.0, 0.0, 0.2], dtype=torch.float64, device="cuda")
+    for steps in (1, 2):
+        v = v0.clone().requires_grad_(True)
+        decisions = []
+        x, _ = rollout(d, x0, v, g, tr, steps, decisions, 1)
+        L = loss_fn(d, x, target, com_t)
+        L.backward()
+        offs = decisions[0][2 * n: 3 * n + 1].cpu().numpy().astype(np.int64)
+        longest = int(np.diff(offs).max())
+        print(f"steps {steps}: longest list {longest}, mean {offs[-1] / n:.1f}")
+        assert longest > 192 and offs[-1] < 192 * n
+        with torch.no_grad():
+            xr, _ = rollout(d, x0, v0, g.detach(), tr.detach(), steps, decisions, 2)
+        assert torch.equal(xr, x.detach())
+        gv = v.grad.clone()
+        eps = 1e-7
+        for k in range(3):
+            dirn = torch.as_tensor(rng.normal(0, 1, v0.shape), dtype=torch.float64).cuda()
+            dirn[:, 3] = 0
+            dirn /= dirn.norm()
+            with torch.no_grad():
+                lp = loss_fn(d, rollout(d, x0, v0 + eps * dirn, g.detach(), tr.detach(), steps, decisions, 2)[0],
+                             target, com_t).item()
+                lm = loss_fn(d, rollout(d, x0, v0 - eps * dirn, g.detach(), tr.detach(), steps, decisions, 2)[0],
+                             target, com_t).item()
+            fd = (lp - lm) / (2 * eps)
+            an = float((gv * dirn).sum())
+            rel = abs(fd - an) / max(abs(fd), 1e-12)
+            print(f"  direction {k}: fd {fd:.9e} adjoint {an:.9e} rel {rel:.2e}")
+            assert rel < 1e-4
+
+
+def test_adjoint_list_overflow_raises(fx):
+    """More pairs than n * 192 in total: the capped lists are flagged and the
+    rollout raises instead of returning a wrong gradient."""
+    import torch
+
+    from flowx_amd.adjoint import rollout
+
+    cfg, d, x0 = _cluster_scene(fx, (0, 0, 0), 400)
+    v0 = torch.zeros_like(x0)
+    g = torch.tensor(-9.81, dtype=torch.float64, device="cuda")
+    tr = torch.zeros(3, dtype=torch.float64, device="cuda")
+    with pytest.raises(RuntimeError, match="overflow"):
+        rollout(d, x0, v0, g, tr, 1, [], 1)
+    assert not d.overflowed()  # reading the flag cleared it
