@@ -262,6 +262,9 @@ def main():
                     help="the reference's literal position correction (diverges at these masses)")
     ap.add_argument("--no-surface", action="store_true")
     ap.add_argument("--workload", default="scale", choices=["scale", "batch", "adjoint"])
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run three plain device-to-device copies of MIB MiB (the reference "
+                         "dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--fd-directions", type=int, default=32)
     ap.add_argument("--fd-checks", default="4:1e-4:1e-8,8:1e-6:1e-9",
                     help="adjoint workload: comma-separated substeps:eps:eps_translation finite-difference checks")
@@ -272,6 +275,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}))
+        return
 
     import __graft_entry__
 

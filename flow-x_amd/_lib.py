@@ -23,7 +23,7 @@ LIB_PATH = PKG_DIR / "libflowx_hip.so"
 CSRC = PKG_DIR / "csrc"
 INCLUDE = REPO_DIR / "include"
 
-HIP_SOURCES = ("fx_scan.hip", "fx_solver.hip", "fx_surface.hip", "fx_batch.hip", "fx_adjoint.hip")
+HIP_SOURCES = ("fx_scan.hip", "fx_solver.hip", "fx_surface.hip", "fx_adjoint.hip")
 
 HIPCC_FLAGS = (
     "--offload-arch=gfx950",

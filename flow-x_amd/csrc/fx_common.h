@@ -204,6 +204,20 @@ struct ProfScope {
     hipLaunchKernelGGL(kernel, grid, block, 0, st, __VA_ARGS__); \
   } while (0)
 
+// Workgroups are dispatched round-robin over the chip's 8 XCDs, each with
+// its own L2.  xcd_block() renumbers the blocks of a 1-D launch so that each
+// XCD runs one contiguous range of them: the neighbour gathers of a sorted
+// particle range then hit lines its own L2 already holds, instead of every
+// L2 pulling the whole position array.  A bijection on [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, G = gridDim.x;
+  const int x = b % kXcds, local = b / kXcds;
+  const int per = (G + kXcds - 1) / kXcds;
+  const int tall = G % kXcds == 0 ? kXcds : G % kXcds;  // XCDs that receive `per` blocks
+  return x < tall ? x * per + local : tall * per + (x - tall) * (per - 1) + local;
+}
+
 inline unsigned grid_for(long long n, int block = kBlock) {
   long long g = (n + block - 1) / block;
   return (unsigned)(g < 1 ? 1 : g);

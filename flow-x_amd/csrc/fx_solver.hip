@@ -825,7 +825,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
                                               float4* __restrict__ pred4_out, Lists L, Scenes S) {
   constexpr bool FUSE = (V & 4) != 0;
   constexpr int MODE = V & 3;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= n) return;
   FxParams p = p_;
   Grid g = g_;
