@@ -6,6 +6,7 @@ The directory is named flow-x_amd; load it as the package ``flowx_amd`` with
 
 from . import _lib
 from .batch import BatchSolver
+from .cache import FrameCache, config_hash
 from .collision import ColliderGrid, grid_geometry
 from .config import (
     DomainSettings,
@@ -23,7 +24,7 @@ from .sph import Solver
 from .surface import MarchingCubes, Surface, SurfaceConfig, resolve
 
 __all__ = [
-    "BatchSolver", "ColliderGrid", "DomainSettings", "MarchingCubes", "Solver", "SolverConfig", "Surface", "SurfaceConfig",
-    "adjoint_config", "animated_config", "batch_configs", "dam_break_config", "grid_geometry",
+    "BatchSolver", "ColliderGrid", "DomainSettings", "FrameCache", "MarchingCubes", "Solver", "SolverConfig", "Surface", "SurfaceConfig",
+    "adjoint_config", "animated_config", "batch_configs", "config_hash", "dam_break_config", "grid_geometry",
     "lattice_config", "resolve", "resolve_config", "scale_config", "seed_positions",
 ]
