@@ -18,7 +18,7 @@ from collections import deque
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, autotune
 from ._lib import FxState
 from .collision import ColliderGrid
 from .config import seed_positions
@@ -43,6 +43,7 @@ class Solver:
         if not torch.cuda.is_available():
             raise RuntimeError("Solver needs a ROCm GPU (torch.cuda.is_available() is False)")
         _lib.lib()  # fail loudly now if the HIP library is missing
+        autotune.apply()  # tuning.json from a previous autotune run, if any
         self.cfg = cfg
         self.device = torch.device(device)
         self.keep_scratch = keep_scratch

@@ -71,3 +71,21 @@ def test_mc_table_of_library_matches_reference_golden(fx):
                   "fx_mc_tables")
     assert np.array_equal(tri, g["tri_table"])
     assert np.array_equal(mask, g["edge_table"])
+
+
+def test_autotune_file_is_applied(fx, tmp_path):
+    """A tuning file written by the autotuner is loaded into the library."""
+    import json
+
+    from flowx_amd import _lib, autotune
+
+    saved = _lib.get_tuning()
+    try:
+        f = tmp_path / "tuning.json"
+        f.write_text(json.dumps({"best": {"block": 128, "list_skin": 0.2}, "results": []}))
+        assert autotune.apply(f) == {"block": 128, "list_skin": 0.2}
+        t = _lib.get_tuning()
+        assert t["block"] == 128 and abs(t["list_skin"] - 0.2) < 1e-7
+        assert autotune.apply(tmp_path / "missing.json") is None
+    finally:
+        _lib.set_tuning(list_skin=saved["list_skin"], block=saved["block"])

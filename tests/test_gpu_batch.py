@@ -94,3 +94,20 @@ def test_batch_graph_replay_matches_eager(fx):
         eager.substep()
         torch.cuda.synchronize()
         assert np.array_equal(graphed.pos.cpu().numpy(), eager.pos.cpu().numpy())
+
+
+def test_autotune_sweep_keeps_results_identical(fx, tmp_path):
+    """The autotuner's sweep (block size x list skin) on a small scene: every
+    setting reproduces the first setting's positions, and the fastest is
+    written and applied."""
+    from flowx_amd import _lib, autotune
+
+    saved = _lib.get_tuning()
+    try:
+        cfg = fx.dam_break_config()
+        out = autotune.tune(fx, cfg, blocks=(128, 256), skins=(0.0, 0.2), warmup=2, steps=3,
+                            path=tmp_path / "t.json")
+        assert len(out["results"]) == 4 and all(r["same_result"] for r in out["results"])
+        assert autotune.apply(tmp_path / "t.json") == out["best"]
+    finally:
+        _lib.set_tuning(list_skin=saved["list_skin"], block=saved["block"])
