@@ -138,6 +138,9 @@ SIGNATURES = {
     "fx_voxelize_cylinder": (
         _I, [_VP, _VP, ctypes.c_float, _VP, ctypes.c_float, ctypes.c_float, _VP, _I, _VP]
     ),
+    "fx_voxelize_mesh": (
+        _I, [_VP, _VP, ctypes.c_float, _VP, _I, _VP, _I, _VP, _VP, ctypes.c_float, _VP, _I, _VP, _VP]
+    ),
     "fx_batch_workspace_bytes": (ctypes.c_size_t, [_VP, _I]),
     "fx_batch_substep": (_I, [_VP, _I, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
     "fx_batch_load_state": (_I, [_VP, _I, _VP, _VP, _VP, _VP, ctypes.c_size_t, _VP]),

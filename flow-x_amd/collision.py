@@ -54,8 +54,9 @@ class ColliderGrid:
         self.grid.zero_()
 
     def add(self, shape, accumulate=True):
-        """OR one analytic shape into the grid.  shape: ("box", lo, hi) |
-        ("sphere", centre, r) | ("cylinder", base, r, height)."""
+        """OR one shape into the grid.  shape: ("box", lo, hi) |
+        ("sphere", centre, r) | ("cylinder", base, r, height) |
+        ("mesh", MeshCollider)."""
         kind = shape[0]
         d, o, v = _i3(self.dims), _f3(self.origin), ctypes.c_float(self.voxel)
         g = ctypes.c_void_p(self.grid.data_ptr())
@@ -69,6 +70,14 @@ class ColliderGrid:
         elif kind == "cylinder":
             _lib.call("fx_voxelize_cylinder", d, o, v, _f3(shape[1]), ctypes.c_float(shape[2]),
                       ctypes.c_float(shape[3]), g, acc, self._stream())
+        elif kind == "mesh":
+            m = shape[1]
+            flag = torch.zeros(1, dtype=torch.int32, device=self.grid.device)
+            _lib.call("fx_voxelize_mesh", d, o, v, ctypes.c_void_p(m.verts.data_ptr()), int(m.verts.shape[0]),
+                      ctypes.c_void_p(m.tris.data_ptr()), int(m.tris.shape[0]), _f3(m.bounds_lo), _f3(m.bounds_hi),
+                      ctypes.c_float(m.ray_eps), g, acc, ctypes.c_void_p(flag.data_ptr()), self._stream())
+            if int(flag.item()):
+                raise RuntimeError("mesh collider: a voxel column crossed more than 128 triangles")
         else:
             raise ValueError(f"unknown collider shape {kind!r}")
 
@@ -104,3 +113,29 @@ def occupancy_cpu(origin, voxel, dims, shapes):
             c, r, hgt = shape[1], shape[2], shape[3]
             occ |= ((x - c[0]) ** 2 + (y - c[1]) ** 2 < r * r) & (z > c[2]) & (z < c[2] + hgt)
     return occ.astype(np.uint8)
+
+
+class MeshCollider:
+    """A triangle mesh placed in the world, ready for fx_voxelize_mesh.
+
+    verts_local: (V, 3) local-space vertices; tris: (F, 3) vertex indices
+    (polygons fanned into triangles by the caller); matrix: 4 x 4 object-to-
+    world transform.  The world bounds are those of the transformed local
+    bounding box (the object's world_bounds), and the ray restart step is
+    the reference's 1e-4 local units expressed along the world +z ray
+    (collision/__init__.py:263-273, :336-346)."""
+
+    def __init__(self, verts_local, tris, matrix=None, device="cuda"):
+        v = np.asarray(verts_local, np.float64).reshape(-1, 3)
+        m = np.eye(4) if matrix is None else np.asarray(matrix, np.float64)
+        world = v @ m[:3, :3].T + m[:3, 3]
+        lo, hi = v.min(0), v.max(0)
+        corners = np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])])
+        cw = corners @ m[:3, :3].T + m[:3, 3]
+        self.bounds_lo, self.bounds_hi = cw.min(0), cw.max(0)
+        inv3 = np.linalg.inv(m[:3, :3])
+        self.ray_eps = 1e-4 / float(np.linalg.norm(inv3 @ np.array([0.0, 0.0, 1.0])))
+        self.verts_world = world.astype(np.float32)
+        self.tris_np = np.asarray(tris, np.int32).reshape(-1, 3)
+        self.verts = torch.as_tensor(self.verts_world).to(device)
+        self.tris = torch.as_tensor(self.tris_np).to(device)

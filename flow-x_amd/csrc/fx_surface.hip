@@ -11,6 +11,8 @@
 // faces, per-face contour segments, loops walked in first-insertion order,
 // reversed fans.  Vertex positions are computed in float64 without
 // contraction and rounded once to float32.
+#include <algorithm>
+
 #include "fx_common.h"
 
 #pragma clang fp contract(off)
@@ -469,6 +471,99 @@ static int voxelize(const int32_t dims[3], const float origin[3], float voxel, c
   return check_launch("voxelize");
 }
 
+// ---- mesh colliders (collision/__init__.py:263-273, :304-367) --------------
+//
+// The reference casts a ray from every voxel centre inside the mesh's world
+// bounds along the world +z axis (the local-space image of it, for a BVH in
+// local space), counts hits - restarting 1e-4 (local units) beyond each hit,
+// at most 64 times - and marks the voxel when the count is odd.  Here one
+// thread owns a column (i, j): it intersects the vertical line through the
+// column's voxel centres with every triangle (world space, in chunks staged in
+// LDS), keeps the hit heights sorted, and replays the reference's hit chain
+// upward from each voxel centre: the first hit at or above it, then each
+// next hit more than ray_eps (world units) beyond the previous one.
+constexpr int kMeshHits = 128;
+
+__global__ void k_fill_u8(uint8_t* __restrict__ a, long long n, uint8_t v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+constexpr int kTriChunk = 512;
+
+__global__ __launch_bounds__(64) void k_voxelize_mesh(int nx, int ny, int i0, int i1, int j0, int j1, int k0,
+                                                      int k1, double ox, double oy, double oz, double voxel,
+                                                      const float* __restrict__ verts,
+                                                      const int32_t* __restrict__ tris, int ntris, int nverts,
+                                                      double eps, uint8_t* __restrict__ grid, int accumulate,
+                                                      int32_t* __restrict__ overflow) {
+  __shared__ float tri_lds[kTriChunk * 9];
+  const int ci = i1 - i0, cols = ci * (j1 - j0);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = c < cols;
+  const int i = i0 + (active ? c % ci : 0), j = j0 + (active ? c / ci : 0);
+  const double x = ox + (i + 0.5) * voxel, y = oy + (j + 0.5) * voxel;
+  double hits[kMeshHits];
+  int nh = 0;
+  bool over = false;
+  for (int base = 0; base < ntris; base += kTriChunk) {
+    const int m = min(kTriChunk, ntris - base);
+    __syncthreads();
+    for (int q = threadIdx.x; q < m * 3; q += blockDim.x) {
+      int v = tris[(size_t)(base + q / 3) * 3 + q % 3];
+      v = min(max(v, 0), nverts - 1);
+      tri_lds[q * 3 + 0] = verts[(size_t)v * 3 + 0];
+      tri_lds[q * 3 + 1] = verts[(size_t)v * 3 + 1];
+      tri_lds[q * 3 + 2] = verts[(size_t)v * 3 + 2];
+    }
+    __syncthreads();
+    if (!active) continue;
+    for (int t = 0; t < m; ++t) {
+      const float* T = tri_lds + t * 9;
+      const double ax = T[0], ay = T[1], az = T[2], bx = T[3], by = T[4], bz = T[5], cx = T[6], cy = T[7],
+                   cz = T[8];
+      // 2-D edge functions of (x, y) against the projected triangle
+      const double area = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+      if (area == 0.0) continue;  // edge-on: the vertical line grazes it
+      const double w0 = (bx - x) * (cy - y) - (by - y) * (cx - x);
+      const double w1 = (cx - x) * (ay - y) - (cy - y) * (ax - x);
+      const double w2 = (ax - x) * (by - y) - (ay - y) * (bx - x);
+      const bool inside = area > 0.0 ? (w0 >= 0.0 && w1 >= 0.0 && w2 >= 0.0) : (w0 <= 0.0 && w1 <= 0.0 && w2 <= 0.0);
+      if (!inside) continue;
+      const double z = (w0 * az + w1 * bz + w2 * cz) / area;
+      if (nh == kMeshHits) {
+        over = true;
+        continue;
+      }
+      int p = nh++;  // insertion keeps the hits sorted
+      while (p > 0 && hits[p - 1] > z) {
+        hits[p] = hits[p - 1];
+        --p;
+      }
+      hits[p] = z;
+    }
+  }
+  if (!active) return;
+  if (over && overflow) atomicOr(overflow, 1);
+  int first = 0;
+  for (int k = k0; k < k1; ++k) {
+    const double z = oz + (k + 0.5) * voxel;
+    while (first < nh && hits[first] < z) ++first;
+    int count = 0;
+    double last = 0.0;
+    for (int h = first; h < nh && count < 64; ++h) {
+      if (count > 0 && hits[h] < last + eps) continue;
+      last = hits[h];
+      ++count;
+    }
+    const size_t q = ((size_t)k * ny + j) * nx + i;
+    if (count & 1)
+      grid[q] = 1;
+    else if (!accumulate)
+      grid[q] = 0;
+  }
+}
+
 }  // namespace fx
 
 using namespace fx;
@@ -590,6 +685,34 @@ int fx_voxelize_cylinder(const int32_t dims[3], const float origin[3], float vox
   if (!base) return fail(FX_EINVAL, "fx_voxelize_cylinder: null base");
   VoxShape sh{2, {base[0], base[1], base[2]}, {0, 0, 0}, radius, height};
   return voxelize(dims, origin, voxel, sh, grid, accumulate, (hipStream_t)stream);
+}
+
+int fx_voxelize_mesh(const int32_t dims[3], const float origin[3], float voxel, const float* verts, int nverts,
+                     const int32_t* tris, int ntris, const float bounds_lo[3], const float bounds_hi[3],
+                     float ray_eps, uint8_t* grid, int accumulate, int32_t* overflow, void* stream) {
+  if (!dims || !origin || !bounds_lo || !bounds_hi || !grid) return fail(FX_EINVAL, "fx_voxelize_mesh: null argument");
+  if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return fail(FX_EINVAL, "fx_voxelize_mesh: dims must be >= 1");
+  if (!(voxel > 0.0f)) return fail(FX_EINVAL, "fx_voxelize_mesh: voxel size must be > 0");
+  if (ntris < 0 || nverts < 0 || (ntris > 0 && (!verts || !tris || nverts == 0)))
+    return fail(FX_EINVAL, "fx_voxelize_mesh: bad mesh arrays");
+  if (!(ray_eps >= 0.0f)) return fail(FX_EINVAL, "fx_voxelize_mesh: ray_eps must be >= 0");
+  hipStream_t st = (hipStream_t)stream;
+  // collision/__init__.py:290-294 _index_range
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = std::max(0, (int)std::floor(((double)bounds_lo[a] - origin[a]) / voxel));
+    hi[a] = std::min(dims[a], (int)std::ceil(((double)bounds_hi[a] - origin[a]) / voxel));
+  }
+  if (!accumulate) {
+    const long long total = (long long)dims[0] * dims[1] * dims[2];
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(total)), dim3(kBlock), 0, st, grid, total, (uint8_t)0);
+  }
+  if (ntris == 0 || lo[0] >= hi[0] || lo[1] >= hi[1] || lo[2] >= hi[2]) return check_launch("fx_voxelize_mesh");
+  const long long cols = (long long)(hi[0] - lo[0]) * (hi[1] - lo[1]);
+  hipLaunchKernelGGL(k_voxelize_mesh, dim3(grid_for(cols, 64)), dim3(64), 0, st, dims[0], dims[1], lo[0], hi[0],
+                     lo[1], hi[1], lo[2], hi[2], (double)origin[0], (double)origin[1], (double)origin[2],
+                     (double)voxel, verts, tris, ntris, nverts, (double)ray_eps, grid, 1, overflow);
+  return check_launch("fx_voxelize_mesh");
 }
 
 }  // extern "C"

@@ -299,6 +299,20 @@ int fx_voxelize_sphere(const int32_t dims[3], const float origin[3], float voxel
 int fx_voxelize_cylinder(const int32_t dims[3], const float origin[3], float voxel,
                          const float base[3], float radius, float height, uint8_t* grid,
                          int accumulate, void* stream);
+/* Mesh collider (collision/__init__.py:263-273 _point_inside, :304-367
+ * _rebuild_grid): voxels whose centre lies inside [bounds_lo, bounds_hi) (the
+ * object's world bounds, as _index_range rounds them) get the parity of the
+ * upward ray's hit chain - hits at or above the centre, each next hit at
+ * least ray_eps beyond the previous, at most 64 - against the triangles.
+ * verts: device float32 nverts x 3 in world space; tris: device int32
+ * ntris x 3.  ray_eps is the reference's 1e-4 local-space restart step in
+ * world units (1e-4 / |inverse(matrix_world)_3x3 * z|).  *overflow (device,
+ * optional) is set when a column has more than 128 hits (the extra hits are
+ * dropped). */
+int fx_voxelize_mesh(const int32_t dims[3], const float origin[3], float voxel, const float* verts,
+                     int nverts, const int32_t* tris, int ntris, const float bounds_lo[3],
+                     const float bounds_hi[3], float ray_eps, uint8_t* grid, int accumulate,
+                     int32_t* overflow, void* stream);
 
 #ifdef __cplusplus
 }

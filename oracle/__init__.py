@@ -392,3 +392,56 @@ def _set_argtypes():
 
 
 os.environ.setdefault("OMP_PROC_BIND", "false")
+
+
+# ---- mesh collider voxelization (numpy) -------------------------------------
+
+
+def voxelize_mesh(dims, origin, voxel, verts_world, tris, bounds_lo, bounds_hi, ray_eps):
+    """collision/__init__.py:263-273 _point_inside and :304-367 _rebuild_grid,
+    restated with the ray in world space: for each voxel centre inside the
+    object's index range (_index_range, :290-294), cast upward along +z; a
+    ray cast returns the nearest triangle hit at or beyond its origin
+    (inclusive point-in-triangle test in the xy projection, edge-on triangles
+    never hit); restart ray_eps beyond each hit; at most 64 casts; odd hit
+    count = occupied.  Returns a uint8 (nz, ny, nx) grid."""
+    nx, ny, nz = (int(d) for d in dims)
+    out = np.zeros((nz, ny, nx), np.uint8)
+    V = np.asarray(verts_world, np.float32).astype(np.float64)
+    T = np.asarray(tris, np.int64).reshape(-1, 3)
+    if len(T) == 0:
+        return out
+    rng = []
+    for a, n in enumerate((nx, ny, nz)):
+        lo = max(0, int(np.floor((float(np.float32(bounds_lo[a])) - float(np.float32(origin[a]))) / float(np.float32(voxel)))))
+        hi = min(n, int(np.ceil((float(np.float32(bounds_hi[a])) - float(np.float32(origin[a]))) / float(np.float32(voxel)))))
+        rng.append((lo, hi))
+    ox, oy, oz = (float(np.float32(c)) for c in origin)
+    vox = float(np.float32(voxel))
+    eps = float(np.float32(ray_eps))
+    A, B, C = V[T[:, 0]], V[T[:, 1]], V[T[:, 2]]
+    area = (B[:, 0] - A[:, 0]) * (C[:, 1] - A[:, 1]) - (B[:, 1] - A[:, 1]) * (C[:, 0] - A[:, 0])
+    for j in range(*rng[1]):
+        y = oy + (j + 0.5) * vox
+        for i in range(*rng[0]):
+            x = ox + (i + 0.5) * vox
+            w0 = (B[:, 0] - x) * (C[:, 1] - y) - (B[:, 1] - y) * (C[:, 0] - x)
+            w1 = (C[:, 0] - x) * (A[:, 1] - y) - (C[:, 1] - y) * (A[:, 0] - x)
+            w2 = (A[:, 0] - x) * (B[:, 1] - y) - (A[:, 1] - y) * (B[:, 0] - x)
+            pos = (area > 0) & (w0 >= 0) & (w1 >= 0) & (w2 >= 0)
+            neg = (area < 0) & (w0 <= 0) & (w1 <= 0) & (w2 <= 0)
+            hit = pos | neg
+            if not hit.any():
+                continue
+            zs = np.sort((w0[hit] * A[hit, 2] + w1[hit] * B[hit, 2] + w2[hit] * C[hit, 2]) / area[hit])
+            for k in range(*rng[2]):
+                start = oz + (k + 0.5) * vox
+                count = 0
+                while count < 64:
+                    nxt = zs[zs >= start]
+                    if len(nxt) == 0:
+                        break
+                    count += 1
+                    start = nxt[0] + eps
+                out[k, j, i] = count & 1
+    return out

@@ -72,3 +72,55 @@ class GpuState:
 
     def get(self, name):
         return getattr(self, name).cpu().numpy()
+
+
+# ---- closed triangle meshes for the mesh-collider tests ----------------------
+
+
+def box_mesh(lo, hi):
+    """Closed axis-aligned box: 8 vertices, 12 outward-wound triangles."""
+    import numpy as np
+
+    (x0, y0, z0), (x1, y1, z1) = lo, hi
+    v = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], np.float64)
+    quads = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)]
+    t = []
+    for a, b, c, d in quads:
+        t += [(a, b, c), (a, c, d)]
+    return v, np.array(t, np.int32)
+
+
+def uv_sphere(centre, r, nu=48, nv=24):
+    """Closed UV sphere (poles as single vertices), fanned into triangles."""
+    import numpy as np
+
+    v = [(0.0, 0.0, -r)]
+    for j in range(1, nv):
+        th = np.pi * j / nv - np.pi / 2
+        for i in range(nu):
+            ph = 2 * np.pi * i / nu
+            v.append((r * np.cos(th) * np.cos(ph), r * np.cos(th) * np.sin(ph), r * np.sin(th)))
+    v.append((0.0, 0.0, r))
+    v = np.array(v) + np.asarray(centre)
+    t = []
+    ring = lambda j, i: 1 + (j - 1) * nu + (i % nu)
+    for i in range(nu):
+        t.append((0, ring(1, i + 1), ring(1, i)))
+        t.append((len(v) - 1, ring(nv - 1, i), ring(nv - 1, i + 1)))
+    for j in range(1, nv - 1):
+        for i in range(nu):
+            a, b, c, d = ring(j, i), ring(j, i + 1), ring(j + 1, i + 1), ring(j + 1, i)
+            t += [(a, b, c), (a, c, d)]
+    return v, np.array(t, np.int32)
+
+
+def rotation(axis, angle):
+    import numpy as np
+
+    axis = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    x, y, z = axis
+    c, s = np.cos(angle), np.sin(angle)
+    return np.array([[c + x * x * (1 - c), x * y * (1 - c) - z * s, x * z * (1 - c) + y * s],
+                     [y * x * (1 - c) + z * s, c + y * y * (1 - c), y * z * (1 - c) - x * s],
+                     [z * x * (1 - c) - y * s, z * y * (1 - c) + x * s, c + z * z * (1 - c)]])

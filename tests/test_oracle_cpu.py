@@ -270,3 +270,81 @@ def test_mc_properties_of_reference_tests(oracle):
     assert _watertight(t)
     v0, t0 = oracle.mc_extract(np.zeros(512, np.float32), (8, 8, 8), (0, 0, 0), 0.1, 0.5)
     assert len(v0) == 0 and len(t0) == 0
+
+
+# ---- mesh collider voxelization (collision/__init__.py:263-273, :304-367) ----
+
+
+def _grid(res=24):
+    from flowx_amd.collision import grid_geometry
+
+    return grid_geometry((-1.0, -1.0, 0.0), (1.0, 1.0, 2.0), res)
+
+
+def _centres(origin, voxel, dims):
+    nx, ny, nz = dims
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return (origin[0] + (i + 0.5) * voxel, origin[1] + (j + 0.5) * voxel, origin[2] + (k + 0.5) * voxel)
+
+
+def test_mesh_voxelization_of_a_closed_box_equals_the_box(fx, oracle):
+    from helpers import box_mesh
+
+    from flowx_amd.collision import occupancy_cpu
+
+    origin, voxel, dims = _grid()
+    lo, hi = (-0.33, -0.21, 0.37), (0.41, 0.52, 1.13)  # faces between voxel centres
+    v, t = box_mesh(lo, hi)
+    got = oracle.voxelize_mesh(dims, origin, voxel, v, t, lo, hi, 1e-4)
+    ref = occupancy_cpu(origin, voxel, dims, [("box", lo, hi)])
+    assert got.sum() > 100 and np.array_equal(got.astype(bool), ref)
+
+
+def test_mesh_voxelization_of_a_rotated_box_matches_the_oriented_box(fx, oracle):
+    from helpers import box_mesh, rotation
+
+    origin, voxel, dims = _grid()
+    v, t = box_mesh((-0.3, -0.2, -0.25), (0.3, 0.2, 0.25))
+    R = rotation((1, 2, 3), 0.7)
+    c = np.array([0.1, -0.05, 1.0])
+    w = v @ R.T + c
+    got = oracle.voxelize_mesh(dims, origin, voxel, w, t, w.min(0), w.max(0), 1e-4)
+    x, y, z = _centres(origin, voxel, dims)
+    p = np.stack([x - c[0], y - c[1], z - c[2]], -1) @ R  # local coordinates
+    inside = (np.abs(p[..., 0]) < 0.3) & (np.abs(p[..., 1]) < 0.2) & (np.abs(p[..., 2]) < 0.25)
+    near = (np.abs(np.abs(p) - [0.3, 0.2, 0.25]) < 1e-6).any(-1)
+    assert got.sum() > 50 and np.array_equal(got.astype(bool)[~near], inside[~near])
+
+
+def test_mesh_voxelization_of_a_sphere_away_from_its_surface(fx, oracle):
+    from helpers import uv_sphere
+
+    origin, voxel, dims = _grid()
+    c, r = np.array([0.05, -0.1, 0.9]), 0.55
+    v, t = uv_sphere(c, r)
+    got = oracle.voxelize_mesh(dims, origin, voxel, v, t, v.min(0), v.max(0), 1e-4)
+    x, y, z = _centres(origin, voxel, dims)
+    d = np.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2)
+    chord = r * (1 - np.cos(np.pi / 24))  # polygonal sphere lies within this of the true one
+    far = np.abs(d - r) > chord
+    assert np.array_equal(got.astype(bool)[far], (d < r)[far])
+
+
+def test_open_mesh_parity_and_restart_epsilon(fx, oracle):
+    """A single horizontal quad: voxel centres below it see one hit (odd:
+    occupied), as the reference's parity test does for non-closed meshes;
+    two coincident quads merge into one hit within the restart step, and
+    two quads 2 eps apart are counted twice."""
+    origin, voxel, dims = _grid(8)
+    quad = np.array([[-0.6, -0.6, 1.0], [0.6, -0.6, 1.0], [0.6, 0.6, 1.0], [-0.6, 0.6, 1.0]])
+    t = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    lo, hi = (-0.6, -0.6, 0.0), (0.6, 0.6, 2.0)
+    one = oracle.voxelize_mesh(dims, origin, voxel, quad, t, lo, hi, 1e-4)
+    x, y, z = _centres(origin, voxel, dims)
+    under = (np.abs(x) < 0.6) & (np.abs(y) < 0.6) & (z < 1.0)
+    assert np.array_equal(one.astype(bool), under)
+    twice = np.concatenate([quad, quad])
+    tt = np.concatenate([t, t + 4])
+    assert np.array_equal(oracle.voxelize_mesh(dims, origin, voxel, twice, tt, lo, hi, 1e-4), one)
+    apart = np.concatenate([quad, quad + [0, 0, 2e-3]])
+    assert oracle.voxelize_mesh(dims, origin, voxel, apart, tt, lo, hi, 1e-3).sum() == 0
