@@ -138,6 +138,8 @@ SIGNATURES = {
     "fx_voxelize_cylinder": (
         _I, [_VP, _VP, ctypes.c_float, _VP, ctypes.c_float, ctypes.c_float, _VP, _I, _VP]
     ),
+    "fx_mc_dedup_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64]),
+    "fx_mc_dedup": (_I, [_VP, ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP, ctypes.c_size_t, _VP]),
     "fx_voxelize_mesh": (
         _I, [_VP, _VP, ctypes.c_float, _VP, _I, _VP, _I, _VP, _VP, ctypes.c_float, _VP, _I, _VP, _VP]
     ),

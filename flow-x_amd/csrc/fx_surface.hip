@@ -564,6 +564,99 @@ __global__ __launch_bounds__(64) void k_voxelize_mesh(int nx, int ny, int i0, in
   }
 }
 
+// ---- surface hand-off (solver/surface.py:278-301 _rebuild_mesh) -------------
+//
+// bmesh refuses a face over a vertex set that already has one (any winding),
+// so the reference keeps the first of each duplicate triangle marching cubes
+// emits where the surface pinches.  Here each triangle's sorted vertex
+// triple goes into an open-addressing hash table whose slot records the
+// smallest triangle index with that triple (atomicMin), and an ordered
+// compaction keeps exactly those triangles.
+constexpr unsigned long long kDedupEmpty = ~0ull;
+
+__device__ __forceinline__ unsigned long long tri_key(const int32_t* t) {
+  unsigned long long a = (unsigned)t[0], b = (unsigned)t[1], c = (unsigned)t[2], x;
+  if (a > b) x = a, a = b, b = x;
+  if (b > c) x = b, b = c, c = x;
+  if (a > b) x = a, a = b, b = x;
+  return a | (b << 21) | (c << 42);
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+__global__ void k_dedup_clear(unsigned long long* __restrict__ keys, int32_t* __restrict__ first, long long cap) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cap) return;
+  keys[i] = kDedupEmpty;
+  first[i] = 0x7fffffff;
+}
+
+__global__ void k_dedup_insert(const int32_t* __restrict__ tris, int n, unsigned long long* __restrict__ keys,
+                               int32_t* __restrict__ first, unsigned long long mask) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const unsigned long long k = tri_key(tris + (size_t)t * 3);
+  unsigned long long slot = mix64(k) & mask;
+  for (;;) {  // the table holds at least 2n slots, so a free or matching slot exists
+    const unsigned long long prev = atomicCAS(&keys[slot], kDedupEmpty, k);
+    if (prev == kDedupEmpty || prev == k) {
+      atomicMin(&first[slot], t);
+      return;
+    }
+    slot = (slot + 1) & mask;
+  }
+}
+
+__global__ void k_dedup_flag(const int32_t* __restrict__ tris, int n, const unsigned long long* __restrict__ keys,
+                             const int32_t* __restrict__ first, unsigned long long mask, int32_t* __restrict__ keep) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const unsigned long long k = tri_key(tris + (size_t)t * 3);
+  unsigned long long slot = mix64(k) & mask;
+  while (keys[slot] != k) slot = (slot + 1) & mask;  // inserted above, so it is found
+  keep[t] = first[slot] == t ? 1 : 0;
+}
+
+__global__ void k_dedup_compact(const int32_t* __restrict__ tris, int n, const int32_t* __restrict__ keep,
+                                const int32_t* __restrict__ off, int32_t* __restrict__ out,
+                                int32_t* __restrict__ count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) count[0] = off[n];
+  if (t >= n || !keep[t]) return;
+  const size_t o = (size_t)off[t] * 3;
+  out[o] = tris[(size_t)t * 3];
+  out[o + 1] = tris[(size_t)t * 3 + 1];
+  out[o + 2] = tris[(size_t)t * 3 + 2];
+}
+
+struct DedupWs {
+  size_t keys, first, keep, off, part, total;
+  unsigned long long cap;
+};
+
+static DedupWs dedup_layout(long long n) {
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  DedupWs w;
+  unsigned long long cap = 64;
+  while (cap < 2ull * (unsigned long long)std::max(n, 1LL)) cap <<= 1;
+  w.cap = cap;
+  size_t o = 0;
+  w.keys = o;  o += al(cap * 8);
+  w.first = o; o += al(cap * 4);
+  w.keep = o;  o += al((size_t)n * 4 + 4);
+  w.off = o;   o += al((size_t)n * 4 + 8);
+  w.part = o;  o += al(scan_partials(n + 1) * 4 + 4);
+  w.total = o;
+  return w;
+}
+
 }  // namespace fx
 
 using namespace fx;
@@ -641,6 +734,33 @@ int fx_mc_extract(const float* field, const int32_t dims[3], const double origin
   hipLaunchKernelGGL(k_mc_counts, dim3(1), dim3(64), 0, st, (const int32_t*)bv_off, (const int32_t*)bt_off,
                      (int)nb, counts);
   return check_launch("fx_mc_extract");
+}
+
+size_t fx_mc_dedup_workspace_bytes(int64_t ntris) { return ntris < 0 ? 0 : dedup_layout(ntris).total; }
+
+int fx_mc_dedup(const int32_t* tris, int64_t ntris, int64_t nverts, int32_t* out, int32_t* count, void* work,
+                size_t work_bytes, void* stream) {
+  if (ntris < 0 || !count || (ntris > 0 && (!tris || !out))) return fail(FX_EINVAL, "fx_mc_dedup: bad arguments");
+  if (ntris > (1LL << 30)) return fail(FX_ERANGE, "fx_mc_dedup: %lld triangles exceed 2^30", (long long)ntris);
+  if (nverts > (1LL << 21)) return fail(FX_ERANGE, "fx_mc_dedup: %lld vertices exceed the 21-bit key", (long long)nverts);
+  const DedupWs w = dedup_layout(ntris);
+  if (!work || work_bytes < w.total) return fail(FX_ENOMEM, "fx_mc_dedup: workspace %zu < %zu", work_bytes, w.total);
+  hipStream_t st = (hipStream_t)stream;
+  char* b = (char*)work;
+  unsigned long long* keys = (unsigned long long*)(b + w.keys);
+  int32_t *first = (int32_t*)(b + w.first), *keep = (int32_t*)(b + w.keep), *off = (int32_t*)(b + w.off),
+          *part = (int32_t*)(b + w.part);
+  const int n = (int)ntris;
+  FX_LAUNCH(k_dedup_clear, dim3(grid_for((long long)w.cap)), dim3(kBlock), st, keys, first, (long long)w.cap);
+  if (n > 0) {
+    FX_LAUNCH(k_dedup_insert, dim3(grid_for(n)), dim3(kBlock), st, tris, n, keys, first, w.cap - 1);
+    FX_LAUNCH(k_dedup_flag, dim3(grid_for(n)), dim3(kBlock), st, tris, n, (const unsigned long long*)keys,
+              (const int32_t*)first, w.cap - 1, keep);
+  }
+  FX_TRY(exclusive_scan_i32(keep, off, n, part, st));
+  FX_LAUNCH(k_dedup_compact, dim3(grid_for(std::max(n, 1))), dim3(kBlock), st, tris, n, (const int32_t*)keep,
+            (const int32_t*)off, out, count);
+  return check_launch("fx_mc_dedup");
 }
 
 // Host copy of the derived case table (edge triples, -1 terminated) and the

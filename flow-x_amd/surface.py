@@ -12,6 +12,7 @@ import ctypes
 import math
 from dataclasses import dataclass
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -112,6 +113,49 @@ class MarchingCubes:
         if want_cases:
             out = out + (self.cases[:ncubes],)
         return out
+
+
+def dedup_triangles(tris, nverts):
+    """The triangles bmesh accepts, in order (solver/surface.py:278-301): a
+    triangle over a vertex set that already has one, in any winding, is
+    dropped.  tris: (T, 3) int32 device tensor; returns a (T', 3) tensor."""
+    tris = tris.contiguous()
+    n = int(tris.shape[0])
+    dev = tris.device
+    work = torch.empty(int(_lib.lib().fx_mc_dedup_workspace_bytes(ctypes.c_int64(n))), dtype=torch.uint8, device=dev)
+    out = torch.empty((max(n, 1), 3), dtype=torch.int32, device=dev)
+    count = torch.zeros(1, dtype=torch.int32, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(_lib.lib().fx_mc_dedup(
+        ctypes.c_void_p(tris.data_ptr()), ctypes.c_int64(n), ctypes.c_int64(int(nverts)),
+        ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(count.data_ptr()),
+        ctypes.c_void_p(work.data_ptr()), ctypes.c_size_t(work.numel()), stream), "fx_mc_dedup")
+    return out[: int(count.item())]
+
+
+def mesh_handoff(verts, tris, domain_matrix=None):
+    """Host buffers for the add-on's mesh rebuild (solver/surface.py:255-301):
+    world-space vertices, the de-duplicated triangles as Blender's loop
+    arrays (``mesh.vertices.foreach_set("co", ...)``,
+    ``mesh.loops.foreach_set("vertex_index", ...)``,
+    ``mesh.polygons.foreach_set("loop_start" / "loop_total", ...)``), the
+    smooth flags, and the surface object's parent-inverse matrix (the inverse
+    of the domain's world matrix, :266-268), which cancels the parent
+    transform so the world-space vertices stay where they are."""
+    kept = dedup_triangles(tris, verts.shape[0]) if tris.shape[0] else tris
+    v = verts.detach().cpu().numpy().astype(np.float32)
+    t = kept.cpu().numpy().astype(np.int32)
+    m = np.eye(4) if domain_matrix is None else np.asarray(domain_matrix, np.float64)
+    return {
+        "co": v.reshape(-1),
+        "vertex_index": t.reshape(-1),
+        "loop_start": np.arange(len(t), dtype=np.int32) * 3,
+        "loop_total": np.full(len(t), 3, np.int32),
+        "use_smooth": np.ones(len(t), bool),
+        "matrix_parent_inverse": np.linalg.inv(m),
+        "vertices": len(v),
+        "triangles": len(t),
+    }
 
 
 class Surface:

@@ -281,6 +281,15 @@ int fx_mc_extract(const float* field, const int32_t dims[3], const double origin
                   int64_t tcap, int64_t* counts, uint8_t* cases, void* work,
                   size_t work_bytes, void* stream);
 
+/* Surface hand-off (solver/surface.py:278-301): the triangles bmesh would
+ * accept, in order - a triangle whose vertex set (any winding) already
+ * appeared earlier is dropped, as bmesh.faces.new rejects it.  tris: device
+ * int32 ntris x 3 with vertex indices < nverts <= 2^21; out: device int32
+ * ntris x 3; *count (device int32) receives the kept count.
+ * work: fx_mc_dedup_workspace_bytes(ntris). */
+size_t fx_mc_dedup_workspace_bytes(int64_t ntris);
+int fx_mc_dedup(const int32_t* tris, int64_t ntris, int64_t nverts, int32_t* out, int32_t* count,
+                void* work, size_t work_bytes, void* stream);
 /* The derived case table behind fx_mc_extract (edge triples per case, -1
  * terminated, 16 per case) and the per-case sign-change edge masks: the
  * TRI_TABLE / EDGE_TABLE of solver/marching_cubes.py:162-173.  Host memory. */

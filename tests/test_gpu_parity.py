@@ -390,3 +390,45 @@ def test_neighbour_lists_are_bit_identical_to_full_stencil(fx, scene, tuning):
         fx._lib.set_tuning(**saved)
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+def _dedup_ref(tris):
+    """First occurrence of each sorted vertex triple, in order (bmesh's
+    faces.new rejecting a face over an existing vertex set)."""
+    seen, keep = set(), []
+    for t in tris:
+        k = tuple(sorted(int(x) for x in t))
+        if k not in seen:
+            seen.add(k)
+            keep.append(t)
+    return np.asarray(keep, np.int32).reshape(-1, 3)
+
+
+def test_mesh_handoff_drops_duplicate_triangles_like_bmesh(fx):
+    import torch
+
+    from flowx_amd.surface import dedup_triangles, mesh_handoff
+
+    rng = np.random.default_rng(5)
+    base = rng.integers(0, 3000, (20000, 3)).astype(np.int32)
+    dup = base[rng.integers(0, len(base), 8000)]
+    perm = np.array([[0, 1, 2], [1, 2, 0], [2, 0, 1], [0, 2, 1], [2, 1, 0], [1, 0, 2]])
+    dup = np.take_along_axis(dup, perm[rng.integers(0, 6, len(dup))], 1)  # any winding / rotation
+    tris = np.concatenate([base, dup])
+    tris = tris[rng.permutation(len(tris))]
+    got = dedup_triangles(torch.as_tensor(tris).cuda(), 3000).cpu().numpy()
+    assert np.array_equal(got, _dedup_ref(tris))
+    assert dedup_triangles(torch.zeros((0, 3), dtype=torch.int32).cuda(), 1).shape[0] == 0
+    # marching cubes on two spheres touching at a point emits real duplicates
+    n = 40
+    g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1)[..., ::-1].astype(np.float64)
+    f = np.maximum(1 - np.linalg.norm(g - [12, 20, 20], axis=-1) / 8, 1 - np.linalg.norm(g - [28, 20, 20], axis=-1) / 8)
+    field = torch.as_tensor(f.astype(np.float32).reshape(-1)).cuda()
+    mc = fx.MarchingCubes()
+    verts, tris = mc.extract(field, (n, n, n), (0.0, 0.0, 0.0), 1.0, 0.5)
+    ref = _dedup_ref(tris.cpu().numpy())
+    h = mesh_handoff(verts, tris, np.diag([2.0, 2.0, 2.0, 1.0]))
+    assert h["triangles"] == len(ref) and np.array_equal(h["vertex_index"].reshape(-1, 3), ref)
+    assert np.allclose(h["matrix_parent_inverse"], np.diag([0.5, 0.5, 0.5, 1.0]))
+    assert h["loop_start"][-1] == 3 * (len(ref) - 1) and h["co"].size == 3 * h["vertices"]
+    print(f"MC two spheres: {tris.shape[0]} triangles, {tris.shape[0] - len(ref)} duplicates dropped")
