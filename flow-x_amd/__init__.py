@@ -21,10 +21,11 @@ from .config import (
     seed_positions,
 )
 from .sph import Solver
-from .surface import MarchingCubes, Surface, SurfaceConfig, resolve
+from .surface import MarchingCubes, Surface, SurfaceConfig, dedup_triangles, mesh_handoff, resolve
+from .timeline import Timeline
 
 __all__ = [
-    "BatchSolver", "ColliderGrid", "DomainSettings", "FrameCache", "MarchingCubes", "Solver", "SolverConfig", "Surface", "SurfaceConfig",
-    "adjoint_config", "animated_config", "batch_configs", "config_hash", "dam_break_config", "grid_geometry",
+    "BatchSolver", "ColliderGrid", "DomainSettings", "FrameCache", "MarchingCubes", "Timeline", "Solver", "SolverConfig", "Surface", "SurfaceConfig",
+    "adjoint_config", "animated_config", "batch_configs", "config_hash", "dam_break_config", "dedup_triangles", "grid_geometry", "mesh_handoff",
     "lattice_config", "resolve", "resolve_config", "scale_config", "seed_positions",
 ]
