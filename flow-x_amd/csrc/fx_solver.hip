@@ -568,6 +568,37 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
                                       int bx, int by, int bz, int t, int n, const Lists& L, LD&& ld,
                                       F&& f, RF&& rf = RF()) {
   const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
+  if (MODE == VISIT_FULL) {
+    // the next row's run bounds are loaded while the current row is scanned
+    auto row = [&](int r, int& c0, int& s, int& e) {
+      const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
+      if (z < 0 || z >= g.cz || y < 0 || y >= g.cy) {
+        c0 = -1;
+        s = e = 0;
+        return;
+      }
+      c0 = (z * g.cy + y) * g.cx + x0;
+      s = off[c0];
+      e = off[c0 + (x1 - x0) + 1];
+    };
+    int c0n, sn, en;
+    row(0, c0n, sn, en);
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+      const int c0 = c0n, s = sn, e = en;
+      if (r < 8) row(r + 1, c0n, sn, en);
+      if (c0 < 0) continue;
+      if (e - s <= max_scan) {
+        scan_run(s, e, ld, f, rf);
+      } else {
+        for (int c = c0; c <= c0 + (x1 - x0); ++c) {
+          const int cs = off[c], ce = min(off[c + 1], cs + max_scan);
+          scan_run(cs, ce, ld, f, rf);
+        }
+      }
+    }
+    return;
+  }
   int full = 0;
   float di = 0.0f;
   if (MODE == VISIT_LIST) {
