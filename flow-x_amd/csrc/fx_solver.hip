@@ -82,7 +82,7 @@ ProfScope::~ProfScope() {
   (void)hipEventRecord(g_prof_recs[slot].b, st);
 }
 
-static Tuning g_tuning = {0.0f, 256};
+static Tuning g_tuning = {0.0f, 64};
 
 Tuning tuning() { return g_tuning; }
 
@@ -1199,7 +1199,11 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
     if (!s.normal) return fail(FX_EINVAL, "surface_tension > 0 needs a normal buffer");
     if (!grid_ready(s.work, n, p.n_cells))
       return fail(FX_EINVAL, "surface tension reads the previous grid: call fx_load_state first");
-    if (n > 0) NLAUNCH(k_normal<VISIT_LIST>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L, Scenes{nullptr, 0});
+    // without lists the one-mode kernels need fewer registers (higher occupancy)
+    if (n > 0 && lists)
+      NLAUNCH(k_normal<VISIT_LIST>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L, Scenes{nullptr, 0});
+    else if (n > 0)
+      NLAUNCH(k_normal<VISIT_FULL>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L, Scenes{nullptr, 0});
   }
   const int iters = p.iterations;
   int cur = iters & 1;  // the last iteration writes buffer 0
@@ -1218,15 +1222,25 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
       NLAUNCH(k_lambda<VISIT_BUILD>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else
       NLAUNCH(k_lambda<VISIT_LIST>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
-    NLAUNCH(k_delta<VISIT_LIST>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
-            w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
-            (float4*)nullptr, L, lists ? (const int32_t*)w.skey : (const int32_t*)nullptr, Scenes{nullptr, 0});
+    if (lists)
+      NLAUNCH(k_delta<VISIT_LIST>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
+              w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
+              (float4*)nullptr, L, (const int32_t*)w.skey, Scenes{nullptr, 0});
+    else
+      NLAUNCH(k_delta<VISIT_FULL>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
+              w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
+              (float4*)nullptr, L, (const int32_t*)nullptr, Scenes{nullptr, 0});
     cur ^= 1;
   }
   if (iters == 0) LAUNCH(k_velocity_s, n, st, n, p.dt, w.pred_s[0], w.pos_s, w.dl_s, w.vel_s);
-  NLAUNCH(k_xsph<4 + VISIT_LIST>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
-          w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred, L,
-          Scenes{nullptr, 0});
+  if (lists)
+    NLAUNCH(k_xsph<4 + VISIT_LIST>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+            w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred, L,
+            Scenes{nullptr, 0});
+  else
+    NLAUNCH(k_xsph<4 + VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+            w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred, L,
+            Scenes{nullptr, 0});
   return check_launch("substep");
 }
 
