@@ -855,6 +855,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
                                               float4* __restrict__ pos_s,
                                               float4* __restrict__ pred4_out, Lists L, Scenes S) {
   constexpr bool FUSE = (V & 4) != 0;
+  constexpr bool SPLIT = (V & 16) != 0;  // dv in sorted order for k_finalize_s
   constexpr int MODE = V & 3;
   const int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -906,6 +907,10 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
   }
   const float f = k.visc * (k.mass * k.poly6);
   const float dvx = f * ax, dvy = f * ay, dvz = f * az;
+  if (SPLIT) {
+    delta4[t] = make_float4(dvx, dvy, dvz, 0.0f);
+    return;
+  }
   const int i = min(max(sidx[t], 0), n - 1);
   if (!FUSE) {
     delta4[i] = make_float4(dvx, dvy, dvz, 0.0f);
@@ -920,6 +925,34 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
   pos4[i] = pf;
   vel4[i] = make_float4(vx, vy, vz, 0.0f);
   if (pos_s) pos_s[t] = pf;
+}
+
+// shaders/sph_finalize.glsl:15-96 over sorted slots, after a SPLIT XSPH pass
+// left dv in sorted order: apply dv, collide, clamp, write the canonical
+// position / velocity (and correction / prediction when asked) and pos_s.
+// Keeping the collider search out of the neighbour pass keeps that pass's
+// register count, and so its occupancy, down (63 instead of 81 VGPRs).
+__global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p, Collider col,
+                                                    const float4* __restrict__ pred_s,
+                                                    const float4* __restrict__ vel_s,
+                                                    const float4* __restrict__ dv_s,
+                                                    const int32_t* __restrict__ sidx,
+                                                    float4* __restrict__ delta4, float4* __restrict__ pos4,
+                                                    float4* __restrict__ vel4, float4* __restrict__ pos_s,
+                                                    float4* __restrict__ pred4_out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float4 pi = pred_s[t], vi = vel_s[t], dv = dv_s[t];
+  const int i = min(max(sidx[t], 0), n - 1);
+  if (delta4) delta4[i] = make_float4(dv.x, dv.y, dv.z, 0.0f);
+  float vx = vi.x + dv.x, vy = vi.y + dv.y, vz = vi.z + dv.z;
+  float px = pi.x, py = pi.y, pz = pi.z;
+  if (pred4_out) pred4_out[i] = make_float4(px, py, pz, 1.0f);
+  finalize_one(p, col, px, py, pz, vx, vy, vz);
+  const float4 pf = make_float4(px, py, pz, 1.0f);
+  pos4[i] = pf;
+  vel4[i] = make_float4(vx, vy, vz, 0.0f);
+  pos_s[t] = pf;
 }
 
 // shaders/sph_normal.glsl:24-81 over the previous grid; pred_s.w = the
@@ -1237,10 +1270,15 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
     NLAUNCH(k_xsph<4 + VISIT_LIST>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
             w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred, L,
             Scenes{nullptr, 0});
-  else
-    NLAUNCH(k_xsph<4 + VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
-            w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred, L,
+  else {
+    // dv into the free ping-pong buffer, then the streaming finalize
+    NLAUNCH(k_xsph<16 + VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+            w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, L,
             Scenes{nullptr, 0});
+    LAUNCH(k_finalize_s, n, st, n, p, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+           (const float4*)w.pred_s[1], w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s,
+           (float4*)s.pred);
+  }
   return check_launch("substep");
 }
 

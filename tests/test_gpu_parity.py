@@ -216,7 +216,7 @@ def test_substep_matches_oracle(fx, oracle, scene):
     A substep chains 8 neighbour passes through a Jacobi iteration, so
     per-pass float32 differences of ~1e-7 grow; the single-substep bar is
     1e-5 * h on positions (with the grid, keys and sort order identical), and
-    longer runs, which are chaotic, are judged on the density-constraint
+    longer runs, which are chaotic, are compared by the density-constraint
     error and fluid-height statistics."""
     cfg, p, st, col = scene["cfg"], scene["p"], scene["st"], scene["col"]
     ref = oracle.State(p, st.pos, st.vel, precision="f32", sort_mode="stable")
