@@ -100,6 +100,37 @@ bool grid_ready(const void* work, int n, int n_cells) {
   return it != g_grids.end() && it->second.first == n && it->second.second == n_cells;
 }
 
+// Which workspaces already hold the normals the next substep's normal pass
+// would compute: the XSPH pass of a substep traverses the same grid and the
+// same final predictions with the same densities as the next substep's
+// normal pass (shaders/sph_normal.glsl:15-21 reads the previous substep's
+// state), so with cohesion on it computes both, into the workspace's
+// normal_next; the next substep copies them into the caller's normal buffer
+// where its normal pass would have written them.  Any other grid build or
+// state load forgets the entry.  Keyed by the normal buffer and the
+// constants the pass uses.
+struct NormalKey {
+  const void* normal;
+  float h, mass;
+};
+static std::unordered_map<const void*, NormalKey> g_normals;
+
+static void mark_normals(const void* work, const void* normal, const FxParams& p) {
+  std::lock_guard<std::mutex> lock(g_grid_mu);
+  g_normals[work] = NormalKey{normal, p.h, p.mass};
+}
+
+static void forget_normals(const void* work) {
+  std::lock_guard<std::mutex> lock(g_grid_mu);
+  g_normals.erase(work);
+}
+
+static bool normals_ready(const void* work, const void* normal, const FxParams& p) {
+  std::lock_guard<std::mutex> lock(g_grid_mu);
+  auto it = g_normals.find(work);
+  return it != g_normals.end() && it->second.normal == normal && it->second.h == p.h && it->second.mass == p.mass;
+}
+
 // ---- constants (shaders/sph_common.glsl:68-91, sph_delta.glsl:44) --------
 float poly6_coef(float h) { return 315.0f / (64.0f * kPi * powf(h, 9.0f)); }
 
@@ -153,6 +184,7 @@ Workspace workspace_layout(const FxParams& p) {
   w.dcell = o;    o += align256(c * 4 + 4);
   w.partial = o;  o += align256(scan_partials((long long)c + 1) * 4 + 4);
   w.flags = o;    o += align256(64 * 4);
+  w.normal_next = o; o += align256(n * 16 + 16);
   w.total = o;
   return w;
 }
@@ -160,6 +192,7 @@ Workspace workspace_layout(const FxParams& p) {
 struct Ws {
   int32_t *cnt, *off, *rank, *key, *sidx, *skey, *partial, *flags;
   float4 *pred_s[2], *pos_s, *vel_s;
+  float4* normal_next;  // the next substep's normals, from the XSPH pass
   float2* dl_s;
   uint64_t* mask;
   int32_t *lcell, *lfull;
@@ -182,6 +215,7 @@ static Ws bind_ws(const Workspace& w, void* base) {
   s.pred_s[1] = (float4*)(b + w.pred_s1);
   s.pos_s = (float4*)(b + w.pos_s);
   s.vel_s = (float4*)(b + w.vel_s);
+  s.normal_next = (float4*)(b + w.normal_next);
   s.dl_s = (float2*)(b + w.dl_s);
   s.mask = (uint64_t*)(b + w.mask);
   s.lcell = (int32_t*)(b + w.lcell);
@@ -853,9 +887,11 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
                                               float4* __restrict__ delta4,
                                               float4* __restrict__ pos4, float4* __restrict__ vel4,
                                               float4* __restrict__ pos_s,
-                                              float4* __restrict__ pred4_out, Lists L, Scenes S) {
+                                              float4* __restrict__ pred4_out, Lists L, Scenes S,
+                                              float4* __restrict__ normal4 = nullptr) {
   constexpr bool FUSE = (V & 4) != 0;
   constexpr bool SPLIT = (V & 16) != 0;  // dv in sorted order for k_finalize_s
+  constexpr bool NORMAL = (V & 32) != 0;  // also the next substep's normal pass (k_normal)
   constexpr int MODE = V & 3;
   const int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -877,6 +913,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
   int bx, by, bz;
   cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f, lap = 0.0f;
   {
 #pragma clang fp contract(fast)
     const float h2 = k.h2;
@@ -897,13 +934,32 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
       const float r2 = c.r2;
       if (r2 < h2 && q != t) {
         const float d = h2 - r2;
-        const float w = d * d * d * __builtin_amdgcn_rcpf(fmaxf(vj.w, 1e-6f));
+        const float inv_rho = __builtin_amdgcn_rcpf(fmaxf(vj.w, 1e-6f));
+        const float w = d * d * d * inv_rho;
         ax += (vj.x - vi.x) * w;
         ay += (vj.y - vi.y) * w;
         az += (vj.z - vi.z) * w;
+        if (NORMAL) {  // k_normal's pair term (pj.w is the same density)
+          const float gsc = d * d * inv_rho;
+          nx += gsc * dx;
+          ny += gsc * dy;
+          nz += gsc * dz;
+          const float r = r2 * __builtin_amdgcn_rsqf(r2);
+          lap += (k.h - r) * inv_rho;
+        }
       }
       return false;
     }, rf);
+  }
+  if (NORMAL) {
+    const float gfac = k.mass * (k.poly6 * -6.0f);
+    nx *= gfac;
+    ny *= gfac;
+    nz *= gfac;
+    lap *= k.mass * k.visc_lap;
+    const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
+    const float curvature = (mag > 1e-4f) ? -lap / mag : 0.0f;
+    normal4[min(max(sidx[t], 0), n - 1)] = make_float4(nx, ny, nz, curvature);
   }
   const float f = k.visc * (k.mass * k.poly6);
   const float dvx = f * ax, dvy = f * ay, dvz = f * az;
@@ -1212,7 +1268,9 @@ static int build_grid(const FxParams& p, FxState& s, const Ws& w, bool do_predic
            s.sorted_idx);
     if (s.key) FX_HIP_CHECK(hipMemcpyAsync(s.key, key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
   }
-  // a new grid invalidates the neighbour lists until a lambda pass rebuilds them
+  // a new grid invalidates the neighbour lists until a lambda pass rebuilds
+  // them, and any normals computed for the next substep
+  forget_normals(s.work);
   FX_TRY(fill(w.flags + 2, 1, kInfBits, st));
   FX_TRY(check_launch("build_grid"));
   mark_grid(s.work, n, nc);
@@ -1233,7 +1291,10 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
     if (!grid_ready(s.work, n, p.n_cells))
       return fail(FX_EINVAL, "surface tension reads the previous grid: call fx_load_state first");
     // without lists the one-mode kernels need fewer registers (higher occupancy)
-    if (n > 0 && lists)
+    if (n > 0 && normals_ready(s.work, s.normal, p)) {
+      // computed by the previous substep's XSPH pass
+      FX_HIP_CHECK(hipMemcpyAsync(s.normal, w.normal_next, (size_t)n * 16, hipMemcpyDeviceToDevice, st));
+    } else if (n > 0 && lists)
       NLAUNCH(k_normal<VISIT_LIST>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L, Scenes{nullptr, 0});
     else if (n > 0)
       NLAUNCH(k_normal<VISIT_FULL>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L, Scenes{nullptr, 0});
@@ -1271,13 +1332,21 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
             w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred, L,
             Scenes{nullptr, 0});
   else {
-    // dv into the free ping-pong buffer, then the streaming finalize
-    NLAUNCH(k_xsph<16 + VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
-            w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, L,
-            Scenes{nullptr, 0});
+    // dv into the free ping-pong buffer, then the streaming finalize; with
+    // cohesion on, the same traversal yields the next substep's normals
+    if (p.surface_tension > 0.0f && s.normal && iters > 0) {
+      NLAUNCH(k_xsph<32 + 16 + VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
+              (const float4*)w.vel_s, w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr,
+              (float4*)nullptr, (float4*)nullptr, L, Scenes{nullptr, 0}, w.normal_next);
+    } else {
+      NLAUNCH(k_xsph<16 + VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+              w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, (float4*)nullptr,
+              L, Scenes{nullptr, 0});
+    }
     LAUNCH(k_finalize_s, n, st, n, p, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
            (const float4*)w.pred_s[1], w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s,
            (float4*)s.pred);
+    if (p.surface_tension > 0.0f && s.normal && iters > 0) mark_normals(s.work, s.normal, p);
   }
   return check_launch("substep");
 }
@@ -1292,6 +1361,7 @@ static int canonical_grid(const FxParams& p, const FxState& s, const Ws& w, hipS
     FX_HIP_CHECK(hipMemcpyAsync(w.skey, s.sorted_key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
   }
   FX_LAUNCH(k_off_from_sorted, dim3(grid_for((long long)n + 1)), dim3(kBlock), st, n, nc, w.skey, w.off);
+  forget_normals(s.work);
   FX_TRY(fill(w.flags + 2, 1, kInfBits, st));
   FX_TRY(check_launch("canonical_grid"));
   mark_grid(s.work, n, nc);

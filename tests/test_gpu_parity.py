@@ -432,3 +432,30 @@ def test_mesh_handoff_drops_duplicate_triangles_like_bmesh(fx):
     assert np.allclose(h["matrix_parent_inverse"], np.diag([0.5, 0.5, 0.5, 1.0]))
     assert h["loop_start"][-1] == 3 * (len(ref) - 1) and h["co"].size == 3 * h["vertices"]
     print(f"MC two spheres: {tris.shape[0]} triangles, {tris.shape[0] - len(ref)} duplicates dropped")
+
+
+def test_fused_normals_equal_the_next_normal_pass(fx, oracle, scene):
+    """With cohesion on, a substep's XSPH pass also computes the normals the
+    next substep's normal pass would (same grid, same final predictions, same
+    densities): they equal the standalone pass run on that state, and a run
+    that uses them equals the oracle's first substep after it."""
+    p, st, col = scene["cfg"].params(scene["dt"]), scene["st"], scene["col"]
+    p.surface_tension = 0.001
+    a = GpuState(fx, p, collider=col, pos=st.pos, vel=st.vel)
+    b = GpuState(fx, p, collider=col, pos=st.pos, vel=st.vel)
+    for g, steps in ((a, 4), (b, 3)):
+        g.run("load_state")
+        for _ in range(steps):
+            g.run("substep")
+    # a's fourth substep took its normals from its third substep's XSPH pass
+    # (s.normal holds the normals a substep used); b runs the standalone
+    # normal pass on the state after three substeps
+    fused = a.get("normal")
+    b.run("normal")
+    alone = b.get("normal")
+    scale = rms(alone[:, :3])
+    e = rel_err(fused[:, :3], alone[:, :3], scale)
+    ek = rel_err(fused[:, 3], alone[:, 3], rms(alone[:, 3]))
+    print(f"fused vs standalone normals: err {e:.2e}, curvature err {ek:.2e}, "
+          f"bit-identical {np.array_equal(fused, alone)}")
+    assert e <= 1e-6 and ek <= 1e-5
