@@ -226,7 +226,7 @@ inline unsigned grid_for(long long n, int block = kBlock) {
 // Workspace layout for the solver (offsets in bytes, 256-B aligned).
 struct Workspace {
   size_t cnt, off, rank, key, sidx, skey, pred_s0, pred_s1, pos_s, vel_s, dl_s, mask, lcell, lfull, disp, dcell,
-      partial, flags, normal_next;
+      partial, flags, normal_next, sdcell;
   size_t total;
 };
 

@@ -185,6 +185,7 @@ Workspace workspace_layout(const FxParams& p) {
   w.partial = o;  o += align256(scan_partials((long long)c + 1) * 4 + 4);
   w.flags = o;    o += align256(64 * 4);
   w.normal_next = o; o += align256(n * 16 + 16);
+  w.sdcell = o;   o += align256(c * 4 + 4);
   w.total = o;
   return w;
 }

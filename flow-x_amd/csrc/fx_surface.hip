@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void k_splat(FxSurface sf, Grid g, float mass,
                                                float h2, float coef, int max_scan, Collider col,
                                                const float4* __restrict__ pos_s,
                                                const int32_t* __restrict__ off,
-                                               const int32_t* __restrict__ dmax_bits, float kr,
+                                               const int32_t* __restrict__ dcell_bits, float kr,
                                                float* __restrict__ field, long long S) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
@@ -169,8 +169,9 @@ __global__ __launch_bounds__(256) void k_splat(FxSurface sf, Grid g, float mass,
   const float pz = sf.lo[2] + (float)gz * sf.spacing;
   int bx, by, bz;
   cell_coord(g, px, py, pz, bx, by, bz);
-  const float R = kr + __int_as_float(*dmax_bits) + 1e-3f * kr;
-  const float R2 = R * R;
+  // dcell_bits[c]: how far any particle of cell c lies outside the cell's
+  // box (the grid is the last substep's, the positions are finalized)
+  const float pad = 1e-3f * kr;
   float acc = 0.0f;
   auto ld = [&](int q) { return pos_s[q]; };
   auto body = [&](int, const float4& pj) {
@@ -183,20 +184,24 @@ __global__ __launch_bounds__(256) void k_splat(FxSurface sf, Grid g, float mass,
     if (z < 0 || z >= g.cz) continue;
     const float zlo = g.loz + (float)z * g.cell;
     const float gzd = axis_gap(pz, zlo, zlo + g.cell);
-    const float rz = R2 - gzd * gzd;
-    if (rz <= 0.0f) continue;
     for (int dy = -2; dy <= 2; ++dy) {
       const int y = by + dy;
       if (y < 0 || y >= g.cy) continue;
       const float ylo = g.loy + (float)y * g.cell;
       const float gyd = axis_gap(py, ylo, ylo + g.cell);
-      const float ryz = rz - gyd * gyd;
-      if (ryz <= 0.0f) continue;
+      const float gyz = gzd * gzd + gyd * gyd;
+      // cell c can hold a particle within kr when its box is within
+      // kr + dcell[c] of the sample; the run spans the first to the last such
+      // cell (the cells between are visited too: their sums add nothing)
+      const int row = (z * g.cy + y) * g.cx;
+      const int wlo = max(bx - 2, 0), whi = min(bx + 2, g.cx - 1);
+      if (off[row + wlo] == off[row + whi + 1]) continue;  // no particle in the window
       int xa = g.cx, xb = -1;
-      for (int x = max(bx - 2, 0); x <= min(bx + 2, g.cx - 1); ++x) {
+      for (int x = wlo; x <= whi; ++x) {
         const float xlo = g.lox + (float)x * g.cell;
         const float gxd = axis_gap(px, xlo, xlo + g.cell);
-        if (gxd * gxd < ryz) {
+        const float R = kr + __int_as_float(dcell_bits[row + x]) + pad;
+        if (gxd * gxd + gyz < R * R) {
           xa = min(xa, x);
           xb = max(xb, x);
         }
@@ -223,11 +228,12 @@ __global__ __launch_bounds__(256) void k_splat(FxSurface sf, Grid g, float mass,
   field[s] = value;
 }
 
-// Gathers the finalized positions into sorted order and bounds how far any
-// particle lies outside the box of its cell (wave max, then one atomic).
+// Gathers the finalized positions into sorted order and bounds, per cell,
+// how far its particles lie outside the cell's box (atomic max into the
+// cell's slot: contention only among one cell's particles).
 __global__ void k_gather_pos(int n, const int32_t* __restrict__ sidx, const int32_t* __restrict__ skey,
                              Grid g, const float4* __restrict__ pos, float4* __restrict__ pos_s,
-                             int32_t* __restrict__ dmax_bits) {
+                             int32_t* __restrict__ dcell_bits) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   float d = 0.0f;
   if (t < n) {
@@ -239,13 +245,14 @@ __global__ void k_gather_pos(int n, const int32_t* __restrict__ sidx, const int3
     const float ex = axis_gap(p.x, xlo, xlo + g.cell), ey = axis_gap(p.y, ylo, ylo + g.cell),
                 ez = axis_gap(p.z, zlo, zlo + g.cell);
     d = sqrtf(ex * ex + ey * ey + ez * ez) * 1.000001f;
+    if (d > 0.0f && c >= 0 && c < g.nc) atomicMax(&dcell_bits[c], __float_as_int(d));
   }
-#pragma unroll
-  for (int k = 32; k >= 1; k >>= 1) d = fmaxf(d, __shfl_xor(d, k, 64));
-  if ((threadIdx.x & 63) == 0 && d > 0.0f) atomicMax(dmax_bits, __float_as_int(d));
 }
 
-__global__ void k_zero_i32(int32_t* a) { *a = 0; }
+__global__ void k_fill_cells(int32_t* __restrict__ a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
 
 // ---- marching cubes -------------------------------------------------------
 constexpr int kMcBlock = 256;
@@ -678,9 +685,9 @@ int fx_splat(const FxParams* p, const FxSurface* surf, const FxState* s, float* 
   const int32_t* sidx = (const int32_t*)(b + wl.sidx);
   const int32_t* off = (const int32_t*)(b + wl.off);
   const int32_t* skey = (const int32_t*)(b + wl.skey);
-  int32_t* dmax = (int32_t*)(b + wl.flags) + 3;
+  int32_t* dmax = (int32_t*)(b + wl.sdcell);
   float4* pos_s = (float4*)(b + wl.pos_s);
-  FX_LAUNCH(k_zero_i32, dim3(1), dim3(1), st, dmax);
+  FX_LAUNCH(k_fill_cells, dim3(grid_for(p->n_cells)), dim3(kBlock), st, dmax, p->n_cells);
   if (p->n > 0)
     FX_LAUNCH(k_gather_pos, dim3(grid_for(p->n)), dim3(kBlock), st, p->n, sidx, skey, make_grid(*p),
               (const float4*)s->pos, pos_s, dmax);
