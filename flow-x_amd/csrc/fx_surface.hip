@@ -13,6 +13,10 @@
 // contraction and rounded once to float32.
 #include <algorithm>
 
+#ifndef FX_SPLAT_M
+#define FX_SPLAT_M 2  // lattice samples per splat thread (1, 2, 4, 8 timed: 2 fastest)
+#endif
+
 #include "fx_common.h"
 
 #pragma clang fp contract(off)
@@ -142,43 +146,49 @@ static constexpr McTables kMcHost = build_mc_tables();
 __constant__ McTables kMc = build_mc_tables();
 
 // ---- splat (shaders/surface_splat.glsl:32-97) ------------------------------
-// One thread per sample (x fastest), gathering the 5x5x5 cells around the
-// sample's clamped cell as contiguous x-runs of sorted slots; pos_s holds the
-// finalized positions in the grid's sorted order.  Cells that cannot hold a
-// particle within the kernel radius are skipped: dmax bounds how far any
-// particle lies outside the box of the cell it was sorted into (the grid is
-// built from pre-constraint positions, shaders/surface_splat.glsl:8-9), so a
-// cell whose box is farther than kr + dmax contributes only zero terms.  The
-// visited terms and their order are the reference's, so the sum is exact.
+// Each sample (x fastest) gathers the 5x5x5 cells around its clamped cell as
+// contiguous x-runs of sorted slots; pos_s holds the finalized positions in
+// the grid's sorted order.  Cells that cannot hold a particle within the
+// kernel radius are skipped: dcell[c] bounds how far cell c's particles lie
+// outside its box (the grid is built from pre-constraint positions,
+// shaders/surface_splat.glsl:8-9), so a cell whose box is farther than
+// kr + dcell[c] contributes only zero terms.  The visited terms and their
+// order are the reference's, so the sum is exact.
 __device__ __forceinline__ float axis_gap(float p, float lo, float hi) {
   return fmaxf(fmaxf(lo - p, p - hi), 0.0f);
 }
 
-__global__ __launch_bounds__(256) void k_splat(FxSurface sf, Grid g, float mass, float rho0,
-                                               float h2, float coef, int max_scan, Collider col,
-                                               const float4* __restrict__ pos_s,
-                                               const int32_t* __restrict__ off,
-                                               const int32_t* __restrict__ dcell_bits, float kr,
-                                               float* __restrict__ field, long long S) {
-  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S) return;
-  const int nx = sf.dims[0], ny = sf.dims[1];
-  const int gx = (int)(s % nx), gy = (int)((s / nx) % ny), gz = (int)(s / ((long long)nx * ny));
-  const float px = sf.lo[0] + (float)gx * sf.spacing;
-  const float py = sf.lo[1] + (float)gy * sf.spacing;
-  const float pz = sf.lo[2] + (float)gz * sf.spacing;
-  int bx, by, bz;
-  cell_coord(g, px, py, pz, bx, by, bz);
-  // dcell_bits[c]: how far any particle of cell c lies outside the cell's
-  // box (the grid is the last substep's, the positions are finalized)
+// shaders/surface_splat.glsl:32-97 for M consecutive samples of one lattice
+// row per thread: they share the stencil rows (same y, z), so each candidate
+// is loaded once and tested against all M.  Each sample still visits its own stencil cells (within 2
+// cells of its own, shaders/surface_splat.glsl:62-82) in the reference's
+// order - cells by increasing x, slots ascending, at most max_scan per cell -
+// so every sum is the one-sample kernel's, bit for bit.
+template <int M>
+__global__ __launch_bounds__(256) void k_splat_m(FxSurface sf, Grid g, float mass, float rho0, float h2,
+                                                 float coef, int max_scan, Collider col,
+                                                 const float4* __restrict__ pos_s,
+                                                 const int32_t* __restrict__ off,
+                                                 const int32_t* __restrict__ dcell_bits, float kr,
+                                                 float* __restrict__ field) {
+  const int nx = sf.dims[0], ny = sf.dims[1], nz = sf.dims[2];
+  const int groups = (nx + M - 1) / M;
+  const long long q0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q0 >= (long long)groups * ny * nz) return;
+  const int gx0 = (int)(q0 % groups) * M, gy = (int)((q0 / groups) % ny), gz = (int)(q0 / ((long long)groups * ny));
+  const int m = min(M, nx - gx0);
+  const float py = sf.lo[1] + (float)gy * sf.spacing, pz = sf.lo[2] + (float)gz * sf.spacing;
+  float px[M], acc[M];
+  int bxs[M];
+  int by = 0, bz = 0;
+#pragma unroll
+  for (int u = 0; u < M; ++u) {
+    px[u] = sf.lo[0] + (float)min(gx0 + u, nx - 1) * sf.spacing;
+    acc[u] = 0.0f;
+    cell_coord(g, px[u], py, pz, bxs[u], by, bz);
+  }
   const float pad = 1e-3f * kr;
-  float acc = 0.0f;
-  auto ld = [&](int q) { return pos_s[q]; };
-  auto body = [&](int, const float4& pj) {
-    const float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
-    const float r2 = dist2_exact(dx, dy, dz);
-    if (r2 < h2) acc = poly6_accum_exact(acc, mass, coef, h2 - r2);
-  };
+  const int wlo = max(bxs[0] - 2, 0), whi = min(bxs[M - 1] + 2, g.cx - 1);
   for (int dz = -2; dz <= 2; ++dz) {
     const int z = bz + dz;
     if (z < 0 || z >= g.cz) continue;
@@ -190,42 +200,66 @@ __global__ __launch_bounds__(256) void k_splat(FxSurface sf, Grid g, float mass,
       const float ylo = g.loy + (float)y * g.cell;
       const float gyd = axis_gap(py, ylo, ylo + g.cell);
       const float gyz = gzd * gzd + gyd * gyd;
-      // cell c can hold a particle within kr when its box is within
-      // kr + dcell[c] of the sample; the run spans the first to the last such
-      // cell (the cells between are visited too: their sums add nothing)
       const int row = (z * g.cy + y) * g.cx;
-      const int wlo = max(bx - 2, 0), whi = min(bx + 2, g.cx - 1);
-      if (off[row + wlo] == off[row + whi + 1]) continue;  // no particle in the window
-      int xa = g.cx, xb = -1;
+      if (off[row + wlo] == off[row + whi + 1]) continue;
+      // per sample: the cells of its stencil that can hold a particle within
+      // kr (kr + the cell's displacement bound), as a slot range [slo, shi)
+      int xa[M], xb[M];
+#pragma unroll
+      for (int u = 0; u < M; ++u) xa[u] = g.cx, xb[u] = -1;
       for (int x = wlo; x <= whi; ++x) {
         const float xlo = g.lox + (float)x * g.cell;
-        const float gxd = axis_gap(px, xlo, xlo + g.cell);
         const float R = kr + __int_as_float(dcell_bits[row + x]) + pad;
-        if (gxd * gxd + gyz < R * R) {
-          xa = min(xa, x);
-          xb = max(xb, x);
+#pragma unroll
+        for (int u = 0; u < M; ++u) {
+          const float gxd = axis_gap(px[u], xlo, xlo + g.cell);
+          if (u < m && x >= bxs[u] - 2 && x <= bxs[u] + 2 && gxd * gxd + gyz < R * R) {
+            xa[u] = min(xa[u], x);
+            xb[u] = max(xb[u], x);
+          }
         }
       }
-      if (xb < xa) continue;
-      const int c0 = (z * g.cy + y) * g.cx + xa, c1 = c0 + (xb - xa);
-      const int a = off[c0], e = off[c1 + 1];
-      if (e - a <= max_scan) {
-        scan_run(a, e, ld, body);
+      int ua = g.cx, ub = -1;
+#pragma unroll
+      for (int u = 0; u < M; ++u) ua = min(ua, xa[u]), ub = max(ub, xb[u]);
+      if (ub < ua) continue;
+      const int A = off[row + ua], E = off[row + ub + 1];
+      int slo[M], shi[M];
+#pragma unroll
+      for (int u = 0; u < M; ++u) {
+        slo[u] = xb[u] < xa[u] ? 0 : off[row + xa[u]];
+        shi[u] = xb[u] < xa[u] ? 0 : off[row + xb[u] + 1];
+      }
+      auto body = [&](int q, const float4& pj) {
+#pragma unroll
+        for (int u = 0; u < M; ++u) {
+          const float dx = px[u] - pj.x, dy2 = py - pj.y, dz2 = pz - pj.z;
+          const float r2 = dist2_exact(dx, dy2, dz2);
+          if (q >= slo[u] && q < shi[u] && r2 < h2) acc[u] = poly6_accum_exact(acc[u], mass, coef, h2 - r2);
+        }
+      };
+      auto ld = [&](int q) { return pos_s[q]; };
+      if (E - A <= max_scan) {
+        scan_run(A, E, ld, body);  // no cell of the run reaches the per-cell cap
       } else {
-        for (int c = c0; c <= c1; ++c) {
-          const int cs = off[c], ce = min(off[c + 1], cs + max_scan);
+        for (int x = ua; x <= ub; ++x) {
+          const int cs = off[row + x], ce = min(off[row + x + 1], cs + max_scan);
           scan_run(cs, ce, ld, body);
         }
       }
     }
   }
-  float value = acc / fmaxf(rho0, 1e-6f);
-  if (col.occ) {
-    int cx, cy, cz;
-    collider_coord(col, px, py, pz, cx, cy, cz);
-    if (collider_occupied(col, cx, cy, cz)) value = 0.0f;
+#pragma unroll
+  for (int u = 0; u < M; ++u) {
+    if (u >= m) break;
+    float value = acc[u] / fmaxf(rho0, 1e-6f);
+    if (col.occ) {
+      int cx, cy, cz;
+      collider_coord(col, px[u], py, pz, cx, cy, cz);
+      if (collider_occupied(col, cx, cy, cz)) value = 0.0f;
+    }
+    field[((long long)gz * ny + gy) * nx + gx0 + u] = value;
   }
-  field[s] = value;
 }
 
 // Gathers the finalized positions into sorted order and bounds, per cell,
@@ -695,9 +729,12 @@ int fx_splat(const FxParams* p, const FxSurface* surf, const FxState* s, float* 
   const float kr = surf->kernel_radius;
   const float coef = poly6_coef(kr);
   Collider col = make_collider(*p, s->collider);
-  FX_LAUNCH(k_splat, dim3(grid_for(S)), dim3(256), st, *surf, make_grid(*p), p->mass, p->rest_density,
+  constexpr int M = FX_SPLAT_M;
+  const long long threads = (long long)((surf->dims[0] + M - 1) / M) * surf->dims[1] * surf->dims[2];
+  FX_LAUNCH(k_splat_m<M>, dim3(grid_for(threads)), dim3(256), st, *surf, make_grid(*p), p->mass, p->rest_density,
             kr * kr, coef, p->max_cell_scan > 0 ? p->max_cell_scan : 512, col, (const float4*)pos_s, off,
-            (const int32_t*)dmax, kr, field, S);
+            (const int32_t*)dmax, kr, field);
+  (void)S;
   return check_launch("fx_splat");
 }
 
