@@ -6,8 +6,9 @@
 // sph_xsph.glsl, sph_finalize.glsl) with two definitions that make it
 // differentiable:
 //   * the neighbour structure of a substep is recorded once, after predict:
-//     every pair within h + skin of the predicted positions, each particle's
-//     list sorted by index.  Every pass of the substep sums over these lists
+//     every pair within h + skin of the predicted positions among the 27
+//     grid cells around the particle's cell, each particle's list sorted by
+//     index.  Every pass of the substep sums over these lists
 //     (pairs beyond h contribute zero, and the kernels are C1 at r = h).
 //   * the collider's voxel frame moves with its translation T: occupancy is
 //     sampled at p - T and the free-voxel centres are shifted by T, so a
@@ -155,9 +156,11 @@ __global__ void k_adj_to_f4(int n, const V3<T>* __restrict__ p, float4* __restri
   out[i] = make_float4((float)q.x, (float)q.y, (float)q.z, 1.0f);
 }
 
-// Each particle's neighbours within the list radius (27 cells of size h
-// around its cell), in CSR form: a count pass, an exclusive scan into noff
-// (n + 1 entries), then a fill pass that sorts each list by particle index.
+// Each particle's neighbours within the list radius among the 27 cells
+// around its cell, in CSR form: a count pass, an exclusive scan into noff
+// (n + 1 entries), a fill pass, then k_adj_sort_lists.  Lists sorted by
+// index are deterministic (the cell runs are in atomic arrival order) and
+// keep the lanes of a wave on nearby neighbours at every list position.
 // The pair total is capped at n * kMaxNbr; beyond it the offsets are clamped
 // and the overflow flag is raised.
 template <class T, bool FILL>
@@ -194,19 +197,67 @@ __global__ void k_adj_lists(int n, Grid g, AdjC<T> c, const V3<T>* __restrict__ 
           }
         }
       }
-  if (!FILL) {
-    count[i] = m;
+  if (!FILL) count[i] = m;
+}
+
+// Sort every list ascending, one wave per list.  Up to kWaveSortMax entries
+// sit in registers (element e = 64 r + lane in v[r], padded with INT_MAX) and
+// pass through a bitonic network: steps of stride >= 64 exchange registers
+// within a lane, shorter strides exchange with lane ^ stride by shuffle.
+// Longer lists are queued for sort_queued_runs.
+constexpr int kWaveSortMax = 256;
+
+__device__ __forceinline__ void cmpx(int& a, int& b, bool asc) {
+  const int lo = min(a, b), hi = max(a, b);
+  a = asc ? lo : hi;
+  b = asc ? hi : lo;
+}
+
+__global__ __launch_bounds__(256) void k_adj_sort_lists(int n, const int32_t* __restrict__ noff,
+                                                        int32_t* __restrict__ nbr, int32_t* __restrict__ queue,
+                                                        int32_t* __restrict__ queue_count) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;  // whole waves
+  const int s = noff[i], m = noff[i + 1] - s;
+  if (m <= 1) return;
+  if (m > kWaveSortMax) {
+    if (lane == 0) queue[atomicAdd(queue_count, 1)] = i;
     return;
   }
-  m = min(m, room);
-  for (int a = 1; a < m; ++a) {
-    const int v = L[a];
-    int b = a - 1;
-    while (b >= 0 && L[b] > v) {
-      L[b + 1] = L[b];
-      --b;
+  const int R = m <= 64 ? 1 : (m <= 128 ? 2 : 4);  // registers in use
+  const int P = 64 * R;
+  int v[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int e = 64 * r + lane;
+    v[r] = e < m ? nbr[s + e] : INT_MAX;
+  }
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j == 128) {
+        cmpx(v[0], v[2], (lane & k) == 0);
+        cmpx(v[1], v[3], ((64 + lane) & k) == 0);
+      } else if (j == 64) {
+        cmpx(v[0], v[1], (lane & k) == 0);
+        if (R == 4) cmpx(v[2], v[3], ((128 + lane) & k) == 0);
+      } else {
+        const bool lower = (lane & j) == 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (r < R) {
+            const int o = __shfl_xor(v[r], j);
+            const bool asc = ((64 * r + lane) & k) == 0;
+            v[r] = (asc == lower) ? min(v[r], o) : max(v[r], o);
+          }
+        }
+      }
     }
-    L[b + 1] = v;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int e = 64 * r + lane;
+    if (e < m) nbr[s + e] = v[r];
   }
 }
 
@@ -873,6 +924,12 @@ static int adj_forward(const FxParams& p, const AdjC<T>& c, const V3<T>* x, cons
     FX_LAUNCH(k_adj_cap, dim3(grid_for(n + 1)), dim3(kBlock), st, n, (long long)n * kMaxNbr, noff, flag);
     ALAUNCH((k_adj_lists<T, true>), n, st, n, g, c, (const V3<T>*)P, (const float4*)pf, (const int32_t*)off,
             (const int32_t*)sidx, (int32_t*)nullptr, (const int32_t*)noff, nbr);
+    // lists past kWaveSortMax queue into rank (dead after the scatter), their
+    // count in part[0] (dead after the scan)
+    FX_LAUNCH(k_adj_zero, dim3(1), dim3(kBlock), st, part, 1);
+    FX_LAUNCH(k_adj_sort_lists, dim3((unsigned)((n + 3) / 4)), dim3(256), st, n, (const int32_t*)noff, nbr, rank,
+              part);
+    FX_TRY(sort_queued_runs(noff, nbr, rank, part, st));
   }
   for (int it = 0; it < I; ++it) {
     V3<T>* Pi = P + (size_t)it * n;
@@ -1055,7 +1112,10 @@ int fx_adjoint_splat(const FxParams* p, const FxSurface* surf, int precision, co
       ALAUNCH(k_adj_key, n, st, n, g, (const float4*)pf, key, cnt, rank);
     }
     FX_TRY(exclusive_scan_i32(cnt, off, q.n_cells, part, st));
-    if (n > 0) ALAUNCH(k_adj_scatter, n, st, n, (const int32_t*)key, (const int32_t*)rank, (const int32_t*)off, sidx);
+    if (n > 0) {
+      ALAUNCH(k_adj_scatter, n, st, n, (const int32_t*)key, (const int32_t*)rank, (const int32_t*)off, sidx);
+      FX_TRY(canon_cells(off, sidx, q.n_cells, rank, part, st));  // deterministic summation order
+    }
     FX_LAUNCH(k_adj_splat<double>, dim3(grid_for(S)), dim3(kBlock), st, S, *surf, g, c, (double)kr, coef,
               (const V3<double>*)x, (const int32_t*)off, (const int32_t*)sidx, (double*)field, sample_mask);
   } else {
@@ -1065,7 +1125,10 @@ int fx_adjoint_splat(const FxParams* p, const FxSurface* surf, int precision, co
       ALAUNCH(k_adj_key, n, st, n, g, (const float4*)pf, key, cnt, rank);
     }
     FX_TRY(exclusive_scan_i32(cnt, off, q.n_cells, part, st));
-    if (n > 0) ALAUNCH(k_adj_scatter, n, st, n, (const int32_t*)key, (const int32_t*)rank, (const int32_t*)off, sidx);
+    if (n > 0) {
+      ALAUNCH(k_adj_scatter, n, st, n, (const int32_t*)key, (const int32_t*)rank, (const int32_t*)off, sidx);
+      FX_TRY(canon_cells(off, sidx, q.n_cells, rank, part, st));  // deterministic summation order
+    }
     FX_LAUNCH(k_adj_splat<float>, dim3(grid_for(S)), dim3(kBlock), st, S, *surf, g, c, kr, (float)coef,
               (const V3<float>*)x, (const int32_t*)off, (const int32_t*)sidx, (float*)field, sample_mask);
   }

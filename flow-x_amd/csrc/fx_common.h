@@ -253,4 +253,15 @@ size_t scan_partials(long long n);
 int exclusive_scan_i32(const int32_t* counts, int32_t* out, long long n, int32_t* partial,
                        hipStream_t st);
 
+// Reorder each cell's run sidx[off[c], off[c+1]) to ascending particle index
+// (the counting sort's atomic ranks leave it in arrival order).  Scratch:
+// big_list receives the cells of more than 128 particles (at most n / 129),
+// big_count one int.
+int canon_cells(const int32_t* off, int32_t* sidx, int n_cells, int32_t* big_list, int32_t* big_count,
+                hipStream_t st);
+// Sort the queued runs data[off[q], off[q+1]), q in queue[0, *queue_count),
+// ascending in place, one workgroup per run.
+int sort_queued_runs(const int32_t* off, int32_t* data, const int32_t* queue, const int32_t* queue_count,
+                     hipStream_t st);
+
 }  // namespace fx

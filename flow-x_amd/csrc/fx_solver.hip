@@ -1233,6 +1233,20 @@ static int fill(int32_t* a, long long n, int32_t v, hipStream_t st) {
   return FX_OK;
 }
 
+int canon_cells(const int32_t* off, int32_t* sidx, int n_cells, int32_t* big_list, int32_t* big_count,
+                hipStream_t st) {
+  if (n_cells <= 0) return FX_OK;
+  FX_TRY(fill(big_count, 1, 0, st));
+  LAUNCH(k_canon_cells, n_cells, st, n_cells, off, sidx, big_list, big_count);
+  return sort_queued_runs(off, sidx, big_list, big_count, st);
+}
+
+int sort_queued_runs(const int32_t* off, int32_t* data, const int32_t* queue, const int32_t* queue_count,
+                     hipStream_t st) {
+  FX_LAUNCH(k_sort_big_cells, dim3(64), dim3(256), st, off, data, queue, queue_count);
+  return FX_OK;
+}
+
 // Counting sort of ws.key (counts already in ws.cnt, ranks in ws.rank) into
 // ws.sidx: scan, scatter, canonicalise.  Writes reference-format ranges if
 // requested.
@@ -1243,9 +1257,7 @@ static int sort_from_counts(const FxParams& p, const FxState& s, const Ws& w, hi
   if (n > 0) {
     LAUNCH(k_scatter, n, st, n, w.key, w.rank, w.off, w.sidx);
     // big-cell queue reuses the rank array (dead after the scatter)
-    FX_TRY(fill(w.flags, 1, 0, st));
-    LAUNCH(k_canon_cells, nc, st, nc, w.off, w.sidx, w.rank, w.flags);
-    FX_LAUNCH(k_sort_big_cells, dim3(64), dim3(256), st, w.off, w.sidx, w.rank, w.flags);
+    FX_TRY(canon_cells(w.off, w.sidx, nc, w.rank, w.flags, st));
   }
   return check_launch("sort_from_counts");
 }

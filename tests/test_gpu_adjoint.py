@@ -188,6 +188,53 @@ def test_adjoint_long_lists_in_dense_cluster(fx):
             assert rel < 1e-4
 
 
+def test_adjoint_recorded_lists_are_sorted_stencil_pairs(fx):
+    """The recorded neighbour lists of a float64 substep over a lattice plus
+    a 300-particle cluster (lists past the 256 entries one wave sorts in
+    registers, so the queued block sort runs too): each list is exactly the
+    set of j != i within h (1 + skin) of i's predicted position whose cell is
+    within one of i's cell on every axis, sorted by index.  Brute force over
+    all pairs on the host: at zero velocity the predicted position is
+    x + dt (g dt), and the host repeats the device's float64 and float32
+    operations one for one, so the sets compare exactly."""
+    import torch
+
+    from flowx_amd.adjoint import rollout
+
+    cfg, d, x0 = _cluster_scene(fx, (20, 20, 8), 300)
+    n = len(x0)
+    decisions = []
+    with torch.no_grad():
+        rollout(d, x0, torch.zeros_like(x0), torch.tensor(-9.81, dtype=torch.float64, device="cuda"),
+                torch.zeros(3, dtype=torch.float64, device="cuda"), 1, decisions, 1)
+    rec = decisions[0].cpu().numpy()
+    offs = rec[2 * n: 3 * n + 1].astype(np.int64)
+    lists = rec[3 * n + 1:]
+    p = d.params
+    dt = float(p.dt)
+    P = x0[:, :3].cpu().numpy().copy()
+    P[:, 2] = P[:, 2] + dt * (0.0 + -9.81 * dt)
+    lo = np.array([p.lo[0], p.lo[1], p.lo[2]], np.float32)
+    cells = np.array([p.cells[0], p.cells[1], p.cells[2]])
+    cell = np.floor((P.astype(np.float32) - lo) / np.float32(p.cell_size)).astype(np.int64)
+    cell = np.clip(cell, 0, cells - 1)
+    R = float(p.h) * (1.0 + d.list_skin)
+    r2 = R * R
+    assert offs[0] == 0 and np.all(np.diff(offs) >= 0) and offs[-1] < 192 * n
+    longest = int(np.diff(offs).max())
+    assert longest > 256, longest
+    for a in range(0, n, 500):
+        b = min(n, a + 500)
+        dd = P[a:b, None, :] - P[None, :, :]
+        dist2 = dd[..., 0] * dd[..., 0] + dd[..., 1] * dd[..., 1] + dd[..., 2] * dd[..., 2]
+        near = np.all(np.abs(cell[a:b, None, :] - cell[None, :, :]) <= 1, axis=2)
+        want = (dist2 < r2) & near
+        want[np.arange(b - a), np.arange(a, b)] = False
+        for i in range(a, b):
+            got = lists[offs[i]:offs[i + 1]]
+            assert np.array_equal(got, np.flatnonzero(want[i - a])), i
+
+
 def test_adjoint_list_overflow_raises(fx):
     """More pairs than n * 192 in total: the capped lists are flagged and the
     rollout raises instead of returning a wrong gradient."""
