@@ -36,7 +36,7 @@ def algorithmic_bytes(kernel, n, c):
     per_particle = {
         "k_lambda": 24 + 8.0 * c / n,       # pred(12) + keys/idx(8) -> rho, lambda(8); cell ranges 8C
         "k_delta": 32 + 8.0 * c / n + 12,   # pred, lambda -> delta, plus the fused apply write (12)
-        "k_xsph": 44 + 8.0 * c / n + 24,    # pred, vel, rho -> dv; fused finalize writes pos, vel
+        "k_xsph": 44 + 8.0 * c / n + 16,    # pred, vel, rho -> dv; the next substep's normal (16)
         "k_normal": 36 + 8.0 * c / n,
         "k_predict_key": 48 + 8,            # pos, vel -> vel, pred; key + rank
     }.get(kernel)
@@ -63,6 +63,44 @@ def read_traffic(kernel):
     if "fetch_bytes" in vals and "write_bytes" in vals:
         return vals["fetch_bytes"] + vals["write_bytes"]
     return None
+
+
+COUNTERS_FILE = ROOT / "profiles" / "substep_scale_1.64M_valu_counters.txt"
+
+
+def read_counters(kernel):
+    """VALU and memory-pipeline figures of `kernel` from the committed
+    rocprofv3 counter summary ('counters <kernel>: key=value ...' lines), or
+    None."""
+    if not COUNTERS_FILE.exists():
+        return None
+    for line in COUNTERS_FILE.read_text().splitlines():
+        if line.startswith(f"counters {kernel}:"):
+            return {k: float(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].split())}
+    return None
+
+
+def stencil_candidates(cfg, pos):
+    """Candidates one full-stencil neighbour pass visits: the sum over
+    particles of the occupancy of the 27 cells around the particle's cell
+    (each cell capped at max_cell_scan), from the cells of `pos` (the next
+    substep's grid differs from it by one predict step)."""
+    import numpy as np
+
+    p = cfg.params(0.0)
+    dims = np.array([p.cells[0], p.cells[1], p.cells[2]], np.int64)
+    lo = np.array([p.lo[0], p.lo[1], p.lo[2]], np.float32)
+    c = np.floor((pos[:, :3].astype(np.float32) - lo) / np.float32(p.cell_size)).astype(np.int64)
+    c = np.clip(c, 0, dims - 1)
+    key = (c[:, 2] * dims[1] + c[:, 1]) * dims[0] + c[:, 0]
+    cnt = np.bincount(key, minlength=int(dims.prod())).reshape(dims[2], dims[1], dims[0])
+    pad = np.pad(np.minimum(cnt, p.max_cell_scan), 1)
+    box = np.zeros_like(cnt)
+    for dz in range(3):
+        for dy in range(3):
+            for dx in range(3):
+                box += pad[dz:dz + dims[2], dy:dy + dims[1], dx:dx + dims[0]]
+    return int((cnt * box).sum())
 
 
 def cpu_baseline(cfg, dt, substeps):
@@ -402,7 +440,7 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-substeps", type=int, default=2)
+    ap.add_argument("--cpu-substeps", type=int, default=12)
     ap.add_argument("--literal-update", action="store_true",
                     help="the reference's literal position correction (diverges at these masses)")
     ap.add_argument("--no-surface", action="store_true")
@@ -501,6 +539,14 @@ def main():
                     "kernel": dominant, "avg_us": round(kernels[dominant]["avg_us"], 2),
                     "bytes_per_particle": round(per_particle, 2)}
 
+    # the neighbour passes against their own bounds: candidates per second
+    # and the counter figures of the committed profile
+    cand = stencil_candidates(cfg, solver.pos.cpu().numpy())
+    nb_kernel = dominant if dominant.split("<")[0] in ("k_lambda", "k_delta", "k_xsph") else "k_lambda<VISIT_FULL>"
+    nb_s = kernels[nb_kernel]["avg_us"] * 1e-6
+    neighbour_pass = {"kernel": nb_kernel, "candidates_per_particle": round(cand / n, 2),
+                      "candidate_pairs_per_s": round(cand / nb_s, 1), "counters": read_counters(nb_kernel)}
+
     surface = None
     if not args.no_surface:
         surf = fx.Surface(solver, cfg.extra["surface_resolution"], 0.5, max_samples=1 << 30)
@@ -539,6 +585,7 @@ def main():
             "particles": n, "cells": c, "dt": dt,
         },
         "roofline": roofline,
+        "neighbour_pass": neighbour_pass,
         "cpu_baseline": base,
         "kernels": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kernels.items()},
         "surface": surface,

@@ -590,8 +590,9 @@ struct Lists {
 
 enum { VISIT_FULL = 0, VISIT_BUILD = 1, VISIT_LIST = 2 };
 // Mode bits for VISIT_FULL: how a lane walks its nine stencil rows (see visit).
-constexpr int VISIT_STREAM = 4;      // one stream; batches run on across row ends
-constexpr int VISIT_ROW_BATCH = 8;   // batches stay in a row, the row's last one masked
+constexpr int VISIT_STREAM = 4;        // one stream per lane; batches run on across row ends
+constexpr int VISIT_MASKED_LOADS = 8;  // with VISIT_STREAM: empty batch slots load nothing
+constexpr int VISIT_ROW_BATCH = 16;    // with VISIT_STREAM: batches stay in a row
 
 struct PV {
   float4 p, v;
@@ -607,22 +608,26 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
                                       F&& f, RF&& rf = RF()) {
   const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
   constexpr int M = MODE & 3;
-  if ((MODE & (VISIT_STREAM | VISIT_ROW_BATCH)) && M == VISIT_FULL) {
+  if ((MODE & VISIT_STREAM) && M == VISIT_FULL) {
     // The nine row runs are loaded up front into a queue S/E that shifts on
-    // every row change (which keeps it in registers), and a lane moves on to
-    // its next row without waiting for the other lanes of its wave.  Per row
-    // (the plain walk below) a wave waits at every row end for the lane with
-    // the longest run - and for its one-at-a-time tail: in a developed flow,
-    // where cell occupancy varies, a wave issues 368 loads for the ~235 a
-    // lane needs.  Candidates reach f in the same order either way.
-    //   VISIT_STREAM: batches of FX_BATCH run on across row ends; the
-    //     per-slot row check costs VALU issue, which pays only with a heavy
-    //     candidate body (XSPH, two gathers: 602 -> 525 us at 1.64M
-    //     particles; lambda 354 -> 383 us).
-    //   VISIT_ROW_BATCH: a batch stays inside one row and the row's last
-    //     batch is masked; one row check per batch (lambda 352 -> 343 us;
-    //     delta 345 -> 356 us and XSPH 525 -> 571 us, so they keep their own).
-    // A lane with a row past max_scan takes the per-cell path below.
+    // every row change (which keeps it in registers), and each lane walks
+    // them without waiting for the rest of its wave at row ends.  Walked row
+    // by row (below), a wave waits at every row end for the lane with the
+    // longest run and its one-at-a-time tail: in a developed flow, where cell
+    // occupancy varies, that is 368 loads per wave for the ~235 a lane needs.
+    // Candidates reach f in the same order either way.
+    //   default: batches of FX_BATCH run on across row ends; the empty slots
+    //     of a lane's last batch re-load its first candidate, which keeps the
+    //     batch's loads unconditional and back to back (with each load behind
+    //     its own branch lambda takes 517 us instead of 343).
+    //   VISIT_MASKED_LOADS: empty slots load nothing (XSPH, whose second
+    //     gather already sits behind the range test: 525 vs 575 us).
+    //   VISIT_ROW_BATCH: a batch stays inside one row, the row's last batch
+    //     masked, one row check per batch instead of one per slot (lambda).
+    // Per pass, substeps 8-248 of the 1.64M scene (us; row by row / stream /
+    // row batches): lambda 351 / 384 / 350, delta 354 / 347 / 360, XSPH
+    // 602 / 525 (masked) / 571.  A lane with a row past max_scan takes the
+    // per-cell path below.
     int S[9], E[9];
     bool big = false;
 #pragma unroll
@@ -634,7 +639,7 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
       E[r] = in ? off[c0 + (x1 - x0) + 1] : 0;
       big |= E[r] - S[r] > max_scan;
     }
-    if (!big) {
+    if (!big && (MODE & VISIT_ROW_BATCH)) {
       int k = S[0], e = E[0], left = 8;
       auto next_row = [&]() {
         while (k >= e && left > 0) {
@@ -648,46 +653,61 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
           --left;
         }
       };
-      if (MODE & VISIT_ROW_BATCH) {
+      next_row();
+#pragma unroll 1
+      while (k < e) {
+        decltype(ld(0)) d[FX_BATCH];
+#pragma unroll
+        for (int u = 0; u < FX_BATCH; ++u) d[u] = ld(min(k + u, e - 1));
+#pragma unroll
+        for (int u = 0; u < FX_BATCH; ++u)
+          if (k + u < e) rf(k + u, d[u]);
+#pragma unroll
+        for (int u = 0; u < FX_BATCH; ++u)
+          if (k + u < e) f(k + u, d[u]);
+        k += FX_BATCH;
         next_row();
+      }
+      return;
+    }
+    if (!big) {
+      int k = S[0], e = E[0], left = 8;
 #pragma unroll 1
-        while (k < e) {
-          decltype(ld(0)) d[FX_BATCH];
+      for (;;) {
+        int q[FX_BATCH];
+        bool v[FX_BATCH];
 #pragma unroll
-          for (int u = 0; u < FX_BATCH; ++u) d[u] = ld(min(k + u, e - 1));
+        for (int u = 0; u < FX_BATCH; ++u) {
+          while (k >= e && left > 0) {
 #pragma unroll
-          for (int u = 0; u < FX_BATCH; ++u)
-            if (k + u < e) rf(k + u, d[u]);
-#pragma unroll
-          for (int u = 0; u < FX_BATCH; ++u)
-            if (k + u < e) f(k + u, d[u]);
-          k += FX_BATCH;
-          next_row();
-        }
-      } else {
-#pragma unroll 1
-        for (;;) {
-          int q[FX_BATCH];
-          bool v[FX_BATCH];
-#pragma unroll
-          for (int u = 0; u < FX_BATCH; ++u) {
-            next_row();
-            v[u] = k < e;
-            q[u] = k;
-            k += v[u] ? 1 : 0;
+            for (int r = 0; r < 8; ++r) {
+              S[r] = S[r + 1];
+              E[r] = E[r + 1];
+            }
+            k = S[0];
+            e = E[0];
+            --left;
           }
-          if (!v[0]) break;
-          decltype(ld(0)) d[FX_BATCH];
+          v[u] = k < e;
+          q[u] = k;
+          k += v[u] ? 1 : 0;
+        }
+        if (!v[0]) break;
+        decltype(ld(0)) d[FX_BATCH];
+        if (MODE & VISIT_MASKED_LOADS) {
 #pragma unroll
           for (int u = 0; u < FX_BATCH; ++u)
             if (v[u]) d[u] = ld(q[u]);
+        } else {
 #pragma unroll
-          for (int u = 0; u < FX_BATCH; ++u)
-            if (v[u]) rf(q[u], d[u]);
-#pragma unroll
-          for (int u = 0; u < FX_BATCH; ++u)
-            if (v[u]) f(q[u], d[u]);
+          for (int u = 0; u < FX_BATCH; ++u) d[u] = ld(v[u] ? q[u] : q[0]);
         }
+#pragma unroll
+        for (int u = 0; u < FX_BATCH; ++u)
+          if (v[u]) rf(q[u], d[u]);
+#pragma unroll
+        for (int u = 0; u < FX_BATCH; ++u)
+          if (v[u]) f(q[u], d[u]);
       }
       return;
     }
@@ -799,7 +819,7 @@ __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restr
     if (ok)
       visit<VISIT_LIST>(g, off, max_scan, bx, by, bz, t, n, L, ld, f, rf);
     else
-      visit<VISIT_FULL | (MODE & (VISIT_STREAM | VISIT_ROW_BATCH))>(g, off, max_scan, bx, by, bz, t, n, L, ld, f, rf);
+      visit<VISIT_FULL | (MODE & (VISIT_STREAM | VISIT_MASKED_LOADS | VISIT_ROW_BATCH))>(g, off, max_scan, bx, by, bz, t, n, L, ld, f, rf);
   } else {
     visit<MODE>(g, off, max_scan, bx, by, bz, t, n, L, ld, f, rf);
   }
@@ -837,7 +857,7 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
 #pragma clang fp contract(fast)
     const float h = k.h, h2 = k.h2, r2l = L.r2_list;
     auto ld = [&](int q) { return pred_s[q]; };
-    visit_auto<MODE | VISIT_ROW_BATCH>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+    visit_auto<MODE | VISIT_STREAM | VISIT_ROW_BATCH>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
       const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       const float r2 = dist2_exact(dx, dy, dz);
       if (r2 < h2) {
@@ -910,7 +930,7 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
       const bool scorr = k.scorr_k > 0.0f && k.w_q > 1e-9f;
       const float inv_wq = scorr ? 1.0f / k.w_q : 0.0f;
       auto ld = [&](int q) { return pred_in[q]; };
-      visit_auto<MODE>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+      visit_auto<MODE | VISIT_STREAM>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2_exact(dx, dy, dz);
         if (r2 < h2 && r2 > 1e-12f && q != t) {
@@ -1018,7 +1038,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
       c.r2 = dist2_exact(pi.x - c.p.x, pi.y - c.p.y, pi.z - c.p.z);
       if (c.r2 < h2 && q != t) c.v = vel_s[q];
     };
-    visit_auto<MODE | VISIT_STREAM>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
+    visit_auto<MODE | VISIT_STREAM | VISIT_MASKED_LOADS>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
       const float4 pj = c.p, vj = c.v;
       const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       const float r2 = c.r2;
