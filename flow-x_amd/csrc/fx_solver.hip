@@ -616,7 +616,7 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
     // longest run and its one-at-a-time tail: in a developed flow, where cell
     // occupancy varies, that is 368 loads per wave for the ~235 a lane needs.
     // Candidates reach f in the same order either way.
-    //   default: batches of FX_BATCH run on across row ends; the empty slots
+    //   default: batches of B run on across row ends; the empty slots
     //     of a lane's last batch re-load its first candidate, which keeps the
     //     batch's loads unconditional and back to back (with each load behind
     //     its own branch lambda takes 517 us instead of 343).
@@ -628,6 +628,10 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
     // row batches): lambda 351 / 384 / 350, delta 354 / 347 / 360, XSPH
     // 602 / 525 (masked) / 571.  A lane with a row past max_scan takes the
     // per-cell path below.
+    // loads in flight per lane (substep at 1.64M particles, batch 3 / 4 / 5 /
+    // 6 / 7 / 8: lambda 356 / 346 / 337 / 340 / 338 / 343 us, delta 352 /
+    // 344 / 341 / 340 / 342 / 349, XSPH 548 / 530 / 543 / 533 / 575 / 563)
+    constexpr int B = (MODE & VISIT_MASKED_LOADS) ? 4 : 6;
     int S[9], E[9];
     bool big = false;
 #pragma unroll
@@ -656,16 +660,16 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
       next_row();
 #pragma unroll 1
       while (k < e) {
-        decltype(ld(0)) d[FX_BATCH];
+        decltype(ld(0)) d[B];
 #pragma unroll
-        for (int u = 0; u < FX_BATCH; ++u) d[u] = ld(min(k + u, e - 1));
+        for (int u = 0; u < B; ++u) d[u] = ld(min(k + u, e - 1));
 #pragma unroll
-        for (int u = 0; u < FX_BATCH; ++u)
+        for (int u = 0; u < B; ++u)
           if (k + u < e) rf(k + u, d[u]);
 #pragma unroll
-        for (int u = 0; u < FX_BATCH; ++u)
+        for (int u = 0; u < B; ++u)
           if (k + u < e) f(k + u, d[u]);
-        k += FX_BATCH;
+        k += B;
         next_row();
       }
       return;
@@ -674,10 +678,10 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
       int k = S[0], e = E[0], left = 8;
 #pragma unroll 1
       for (;;) {
-        int q[FX_BATCH];
-        bool v[FX_BATCH];
+        int q[B];
+        bool v[B];
 #pragma unroll
-        for (int u = 0; u < FX_BATCH; ++u) {
+        for (int u = 0; u < B; ++u) {
           while (k >= e && left > 0) {
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
@@ -693,20 +697,20 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
           k += v[u] ? 1 : 0;
         }
         if (!v[0]) break;
-        decltype(ld(0)) d[FX_BATCH];
+        decltype(ld(0)) d[B];
         if (MODE & VISIT_MASKED_LOADS) {
 #pragma unroll
-          for (int u = 0; u < FX_BATCH; ++u)
+          for (int u = 0; u < B; ++u)
             if (v[u]) d[u] = ld(q[u]);
         } else {
 #pragma unroll
-          for (int u = 0; u < FX_BATCH; ++u) d[u] = ld(v[u] ? q[u] : q[0]);
+          for (int u = 0; u < B; ++u) d[u] = ld(v[u] ? q[u] : q[0]);
         }
 #pragma unroll
-        for (int u = 0; u < FX_BATCH; ++u)
+        for (int u = 0; u < B; ++u)
           if (v[u]) rf(q[u], d[u]);
 #pragma unroll
-        for (int u = 0; u < FX_BATCH; ++u)
+        for (int u = 0; u < B; ++u)
           if (v[u]) f(q[u], d[u]);
       }
       return;
