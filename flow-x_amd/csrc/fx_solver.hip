@@ -106,18 +106,21 @@ bool grid_ready(const void* work, int n, int n_cells) {
 // normal pass (shaders/sph_normal.glsl:15-21 reads the previous substep's
 // state), so with cohesion on it computes both, into the workspace's
 // normal_next; the next substep copies them into the caller's normal buffer
-// where its normal pass would have written them.  Any other grid build or
+// where its normal pass would have written them.  Between two substeps of one
+// fx_step call nobody reads the normal buffer, so there the XSPH pass writes
+// straight into it (in_place) and the copy goes.  Any other grid build or
 // state load forgets the entry.  Keyed by the normal buffer and the
 // constants the pass uses.
 struct NormalKey {
   const void* normal;
   float h, mass;
+  bool in_place;
 };
 static std::unordered_map<const void*, NormalKey> g_normals;
 
-static void mark_normals(const void* work, const void* normal, const FxParams& p) {
+static void mark_normals(const void* work, const void* normal, const FxParams& p, bool in_place) {
   std::lock_guard<std::mutex> lock(g_grid_mu);
-  g_normals[work] = NormalKey{normal, p.h, p.mass};
+  g_normals[work] = NormalKey{normal, p.h, p.mass, in_place};
 }
 
 static void forget_normals(const void* work) {
@@ -125,10 +128,13 @@ static void forget_normals(const void* work) {
   g_normals.erase(work);
 }
 
-static bool normals_ready(const void* work, const void* normal, const FxParams& p) {
+// 0: not computed, 1: in the workspace's normal_next, 2: already in `normal`
+static int normals_ready(const void* work, const void* normal, const FxParams& p) {
   std::lock_guard<std::mutex> lock(g_grid_mu);
   auto it = g_normals.find(work);
-  return it != g_normals.end() && it->second.normal == normal && it->second.h == p.h && it->second.mass == p.mass;
+  if (it == g_normals.end() || it->second.normal != normal || it->second.h != p.h || it->second.mass != p.mass)
+    return 0;
+  return it->second.in_place ? 2 : 1;
 }
 
 // ---- constants (shaders/sph_common.glsl:68-91, sph_delta.glsl:44) --------
@@ -1122,7 +1128,7 @@ __global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p, Collider 
   const float4 pf = make_float4(px, py, pz, 1.0f);
   pos4[i] = pf;
   vel4[i] = make_float4(vx, vy, vz, 0.0f);
-  pos_s[t] = pf;
+  if (pos_s) pos_s[t] = pf;
 }
 
 // shaders/sph_normal.glsl:24-81 over the previous grid; pred_s.w = the
@@ -1403,7 +1409,11 @@ static int build_grid(const FxParams& p, FxState& s, const Ws& w, bool do_predic
   return FX_OK;
 }
 
-static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
+// last: this substep's state is what the caller sees next.  Earlier
+// substeps of one fx_step call leave out the outputs the next substep
+// overwrites before anything reads them: the canonical prediction, lambda and
+// correction, and the sorted final positions (read by the surface).
+static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, bool last_in_call = true) {
   const int n = p.n;
   Grid g = make_grid(p);
   Consts k = make_consts(p);
@@ -1417,9 +1427,12 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
     if (!grid_ready(s.work, n, p.n_cells))
       return fail(FX_EINVAL, "surface tension reads the previous grid: call fx_load_state first");
     // without lists the one-mode kernels need fewer registers (higher occupancy)
-    if (n > 0 && normals_ready(s.work, s.normal, p)) {
+    const int ready = n > 0 ? normals_ready(s.work, s.normal, p) : 0;
+    if (ready == 1) {
       // computed by the previous substep's XSPH pass
       FX_HIP_CHECK(hipMemcpyAsync(s.normal, w.normal_next, (size_t)n * 16, hipMemcpyDeviceToDevice, st));
+    } else if (ready == 2) {
+      // written in place by the previous substep's XSPH pass
     } else if (n > 0 && lists)
       NLAUNCH(k_normal<VISIT_LIST>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L, Scenes{nullptr, 0});
     else if (n > 0)
@@ -1427,7 +1440,13 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
   }
   const int iters = p.iterations;
   int cur = iters & 1;  // the last iteration writes buffer 0
-  FX_TRY(build_grid(p, s, w, true, cur, st));
+  {
+    // the reference-format grid (keys, sorted keys and indices, cell ranges)
+    // is published for the caller only by the call's last substep
+    FxState sg = s;
+    if (!last_in_call) sg.key = sg.sorted_key = sg.sorted_idx = sg.cell_start = sg.cell_end = nullptr;
+    FX_TRY(build_grid(p, sg, w, true, cur, st));
+  }
   if (n == 0) return check_launch("substep");
   if (lists && iters > 0) {
     FX_TRY(fill(w.dcell, p.n_cells, 0, st));
@@ -1435,7 +1454,7 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
   }
   for (int it = 0; it < iters; ++it) {
     const bool last = it == iters - 1;
-    float4* lam_out = (last && s.lambda) ? (float4*)s.lambda : (float4*)nullptr;
+    float4* lam_out = (last && last_in_call && s.lambda) ? (float4*)s.lambda : (float4*)nullptr;
     if (!lists)
       NLAUNCH(k_lambda<VISIT_FULL>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else if (it == 0)
@@ -1453,26 +1472,30 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
     cur ^= 1;
   }
   if (iters == 0) LAUNCH(k_velocity_s, n, st, n, p.dt, w.pred_s[0], w.pos_s, w.dl_s, w.vel_s);
+  float4* delta_out = last_in_call ? (float4*)s.delta : nullptr;
+  float4* pred_out = last_in_call ? (float4*)s.pred : nullptr;
+  float4* pos_s_out = last_in_call ? w.pos_s : nullptr;
   if (lists)
     NLAUNCH(k_xsph<4 + VISIT_LIST>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
-            w.off, w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s, (float4*)s.pred, L,
+            w.off, w.sidx, delta_out, (float4*)s.pos, (float4*)s.vel, pos_s_out, pred_out, L,
             Scenes{nullptr, 0});
   else {
     // dv into the free ping-pong buffer, then the streaming finalize; with
     // cohesion on, the same traversal yields the next substep's normals
-    if (p.surface_tension > 0.0f && s.normal && iters > 0) {
+    const bool fuse_normals = p.surface_tension > 0.0f && s.normal && iters > 0;
+    if (fuse_normals) {
       NLAUNCH(k_xsph<32 + 16 + VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
               (const float4*)w.vel_s, w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr,
-              (float4*)nullptr, (float4*)nullptr, L, Scenes{nullptr, 0}, w.normal_next);
+              (float4*)nullptr, (float4*)nullptr, L, Scenes{nullptr, 0},
+              last_in_call ? w.normal_next : (float4*)s.normal);
     } else {
       NLAUNCH(k_xsph<16 + VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
               w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, (float4*)nullptr,
               L, Scenes{nullptr, 0});
     }
     LAUNCH(k_finalize_s, n, st, n, p, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
-           (const float4*)w.pred_s[1], w.sidx, (float4*)s.delta, (float4*)s.pos, (float4*)s.vel, w.pos_s,
-           (float4*)s.pred);
-    if (p.surface_tension > 0.0f && s.normal && iters > 0) mark_normals(s.work, s.normal, p);
+           (const float4*)w.pred_s[1], w.sidx, delta_out, (float4*)s.pos, (float4*)s.vel, pos_s_out, pred_out);
+    if (fuse_normals) mark_normals(s.work, s.normal, p, !last_in_call);
   }
   return check_launch("substep");
 }
@@ -1846,7 +1869,12 @@ int fx_substep(const FxParams* p, FxState* s, void* stream) {
 
 int fx_step(const FxParams* p, FxState* s, int substeps, void* stream) {
   if (substeps < 0) return fail(FX_EINVAL, "substeps < 0");
-  for (int i = 0; i < substeps; ++i) FX_TRY(fx_substep(p, s, stream));
+  if (substeps == 0) return FX_OK;
+  FX_TRY(validate(p, s, true));
+  if (p->n > 0 && (!s->pos || !s->vel)) return fail(FX_EINVAL, "substep needs pos and vel");
+  if (p->n > 0 && !s->pred) return fail(FX_EINVAL, "substep needs a pred buffer");
+  Ws w = bind_ws(workspace_layout(*p), s->work);
+  for (int i = 0; i < substeps; ++i) FX_TRY(substep(*p, *s, w, (hipStream_t)stream, i == substeps - 1));
   return FX_OK;
 }
 

@@ -434,6 +434,41 @@ def test_mesh_handoff_drops_duplicate_triangles_like_bmesh(fx):
     print(f"MC two spheres: {tris.shape[0]} triangles, {tris.shape[0] - len(ref)} duplicates dropped")
 
 
+def test_step_equals_single_substeps(fx, scene):
+    """fx_step(K) leaves out, in all but its last substep, the outputs the
+    next substep overwrites before anything reads them (canonical prediction,
+    lambda, correction, the reference-format grid, the sorted final
+    positions) and lets the fused XSPH pass write the next normals in place:
+    every buffer the caller sees after the call is bit-identical to K single
+    fx_substep calls - cohesion off and on, full stencil and neighbour lists -
+    and a second call carries on identically."""
+    import ctypes
+
+    L = fx._lib
+    p, st, col = scene["cfg"].params(scene["dt"]), scene["st"], scene["col"]
+    names = ("pos", "vel", "pred", "lam", "delta", "normal", "key", "sorted_key", "sorted_idx", "cell_start",
+             "cell_end")
+    saved = L.get_tuning()
+    try:
+        for skin in (0.0, 0.2):
+            L.set_tuning(list_skin=skin, block=saved["block"])
+            for sigma in (0.0, 0.001):
+                p.surface_tension = sigma
+                a = GpuState(fx, p, collider=col, pos=st.pos, vel=st.vel)
+                b = GpuState(fx, p, collider=col, pos=st.pos, vel=st.vel)
+                a.run("load_state")
+                b.run("load_state")
+                for call in range(2):
+                    L.check(L.lib().fx_step(ctypes.byref(p), ctypes.byref(a.state), 3, a.stream()), "fx_step")
+                    a.torch.cuda.synchronize()
+                    for _ in range(3):
+                        b.run("substep")
+                    for name in names:
+                        assert np.array_equal(a.get(name), b.get(name)), (skin, sigma, call, name)
+    finally:
+        L.set_tuning(list_skin=saved["list_skin"], block=saved["block"])
+
+
 def test_fused_normals_equal_the_next_normal_pass(fx, oracle, scene):
     """With cohesion on, a substep's XSPH pass also computes the normals the
     next substep's normal pass would (same grid, same final predictions, same
