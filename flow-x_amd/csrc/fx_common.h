@@ -227,6 +227,9 @@ inline unsigned grid_for(long long n, int block = kBlock) {
 struct Workspace {
   size_t cnt, off, rank, key, sidx, skey, pred_s0, pred_s1, pos_s, vel_s, dl_s, mask, lcell, lfull, disp, dcell,
       partial, flags, normal_next, sdcell;
+  // views into the neighbour-list masks (unused while the lists are off):
+  // the state carried in sorted order between the substeps of one call
+  size_t pair64, pos_r, vel_r;
   size_t total;
 };
 

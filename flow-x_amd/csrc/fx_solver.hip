@@ -18,6 +18,7 @@
 // float32 ulps of each term.
 #include <cstdarg>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <map>
 #include <unordered_map>
@@ -183,7 +184,7 @@ Workspace workspace_layout(const FxParams& p) {
   w.pos_s = o;    o += align256(n * 16 + 16);
   w.vel_s = o;    o += align256(n * 16 + 16);
   w.dl_s = o;     o += align256(n * 8 + 8);
-  w.mask = o;     o += align256(n * 8 * 9 + 8);
+  w.mask = o;     o += std::max(align256(n * 8 * 9 + 8), align256(n * 8) + align256(n * 16) + align256(n * 16 + 16));
   w.lcell = o;    o += align256(n * 4 + 4);
   w.lfull = o;    o += align256(n * 4 + 4);
   w.disp = o;     o += align256(n * 4 + 4);
@@ -192,6 +193,10 @@ Workspace workspace_layout(const FxParams& p) {
   w.flags = o;    o += align256(64 * 4);
   w.normal_next = o; o += align256(n * 16 + 16);
   w.sdcell = o;   o += align256(c * 4 + 4);
+  // 8 + 16 + 16 bytes per particle inside the 72 of the masks
+  w.pair64 = w.mask;
+  w.pos_r = w.pair64 + align256(n * 8);
+  w.vel_r = w.pos_r + align256(n * 16);
   w.total = o;
   return w;
 }
@@ -200,6 +205,8 @@ struct Ws {
   int32_t *cnt, *off, *rank, *key, *sidx, *skey, *partial, *flags;
   float4 *pred_s[2], *pos_s, *vel_s;
   float4* normal_next;  // the next substep's normals, from the XSPH pass
+  uint64_t* pair64;     // (particle << 32 | previous slot), sorted order
+  float4 *pos_r, *vel_r;  // final state by slot, between substeps of a call
   float2* dl_s;
   uint64_t* mask;
   int32_t *lcell, *lfull;
@@ -223,6 +230,9 @@ static Ws bind_ws(const Workspace& w, void* base) {
   s.pos_s = (float4*)(b + w.pos_s);
   s.vel_s = (float4*)(b + w.vel_s);
   s.normal_next = (float4*)(b + w.normal_next);
+  s.pair64 = (uint64_t*)(b + w.pair64);
+  s.pos_r = (float4*)(b + w.pos_r);
+  s.vel_r = (float4*)(b + w.vel_r);
   s.dl_s = (float2*)(b + w.dl_s);
   s.mask = (uint64_t*)(b + w.mask);
   s.lcell = (int32_t*)(b + w.lcell);
@@ -415,12 +425,111 @@ __global__ void k_scatter(int n, const int32_t* __restrict__ key, const int32_t*
   sidx[off[k] + rank[i]] = i;
 }
 
-// Canonicalise each cell's run to ascending particle index.  Runs up to
-// kSmallCell are insertion-sorted by their own thread; longer runs are queued
-// for k_sort_big_cells.
-constexpr int kSmallCell = 128;
+// ---- resident grid build (substeps after the first of one fx_step call) ----
+// The state enters in the previous substep's sorted order: slot t holds
+// particle pid[t], its final position pos_r[t] and velocity vel_r[t], and
+// the normal this substep uses nrm_r[t].  Predict runs per slot; the
+// predicted velocity is not stored (the velocity pass recomputes it from the
+// positions, shaders/sph_velocity.glsl:12-24).  nrm_canon (the call's last
+// substep only) receives the canonical normals the caller's normal buffer
+// holds after the substep.
+__global__ __launch_bounds__(256) void k_predict_resident(FxParams p, Grid g, const float4* __restrict__ pos_r,
+                                                          const float4* __restrict__ vel_r,
+                                                          const float4* __restrict__ nrm_r,
+                                                          float4* __restrict__ pred_r, int32_t* __restrict__ key,
+                                                          int32_t* __restrict__ cnt, int32_t* __restrict__ rank,
+                                                          const int32_t* __restrict__ pid,
+                                                          float4* __restrict__ nrm_canon) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= p.n) return;
+  const float4 nrm = (p.surface_tension > 0.0f && nrm_r) ? nrm_r[t] : make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 vo, pp;
+  predict_one(p, pos_r[t], vel_r[t], nrm, vo, pp);
+  pred_r[t] = pp;
+  int x, y, z;
+  cell_coord(g, pp.x, pp.y, pp.z, x, y, z);
+  const int k = cell_index(g, x, y, z);
+  key[t] = k;
+  rank[t] = atomicAdd(&cnt[k], 1);
+  if (nrm_canon) nrm_canon[min(max(pid[t], 0), p.n - 1)] = nrm;
+}
 
-__global__ void k_canon_cells(int nc, const int32_t* __restrict__ off, int32_t* __restrict__ sidx,
+// Counting-sort scatter of (particle << 32 | previous slot) pairs.
+__global__ void k_scatter_pairs(int n, const int32_t* __restrict__ key, const int32_t* __restrict__ rank,
+                                const int32_t* __restrict__ off, const int32_t* __restrict__ pid,
+                                uint64_t* __restrict__ pairs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  pairs[off[key[t]] + rank[t]] = ((uint64_t)(uint32_t)pid[t] << 32) | (uint32_t)t;
+}
+
+// The gather of a resident build: slot u of the new order reads its previous
+// slot (mostly a nearby one: particles rarely change cells), and records the
+// particle in sidx (plus the caller's reference-format copies when given).
+__global__ void k_gather_pairs(int n, const uint64_t* __restrict__ pairs, const int32_t* __restrict__ key_r,
+                               const float4* __restrict__ pred_r, const float4* __restrict__ pos_r,
+                               float4* __restrict__ pred_s, float4* __restrict__ pos_s,
+                               float2* __restrict__ dl_s, int32_t* __restrict__ skey_out,
+                               int32_t* __restrict__ skey_int, int32_t* __restrict__ sidx,
+                               int32_t* __restrict__ sidx_out, int32_t* __restrict__ key_canon) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const uint64_t e = pairs[u];
+  const int pid = min(max((int)(e >> 32), 0), n - 1);
+  const int src = min(max((int)(uint32_t)e, 0), n - 1);
+  float4 q = pred_r[src];
+  q.w = 0.0f;
+  pred_s[u] = q;
+  pos_s[u] = pos_r[src];
+  dl_s[u] = make_float2(0.0f, 0.0f);
+  const int k = key_r[src];
+  skey_int[u] = k;
+  if (skey_out) skey_out[u] = k;
+  sidx[u] = pid;
+  if (sidx_out) sidx_out[u] = pid;
+  if (key_canon) key_canon[pid] = k;
+}
+
+// Canonicalise each cell's run to ascending particle index.  Runs of up to
+// kSmallCell go through a bitonic network in registers, every load issued at
+// once; longer runs are queued for k_sort_big_cells.  A run is sorted by its
+// own lane, so the kernel lasts as long as its slowest lane: with a
+// per-thread insertion sort (dependent loads, quadratic in the run) a few
+// dense cells set the time - 73 us at 1.64M particles for 8-byte pairs, 43 us
+// at 25,600 particles.
+constexpr int kSmallCell = 32;
+constexpr int kBigCellBlocks = 256;  // workgroups striding over the queued runs
+
+template <int N, class T>
+__device__ __forceinline__ void sort_run_in_registers(T* __restrict__ run, int m) {
+  T a[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) a[u] = run[min(u, m - 1)];
+#pragma unroll
+  for (int u = 0; u < N; ++u)
+    if (u >= m) a[u] = std::numeric_limits<T>::max();
+#pragma unroll
+  for (int kk = 2; kk <= N; kk <<= 1)
+#pragma unroll
+    for (int j = kk >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool asc = (i & kk) == 0;
+          const T x = a[i], y = a[l];
+          const bool sw = asc ? (x > y) : (x < y);
+          a[i] = sw ? y : x;
+          a[l] = sw ? x : y;
+        }
+      }
+#pragma unroll
+  for (int u = 0; u < N; ++u)
+    if (u < m) run[u] = a[u];
+}
+
+template <class T>
+__global__ void k_canon_cells(int nc, const int32_t* __restrict__ off, T* __restrict__ sidx,
                               int32_t* __restrict__ big_list, int32_t* __restrict__ big_count) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
@@ -430,23 +539,19 @@ __global__ void k_canon_cells(int nc, const int32_t* __restrict__ off, int32_t* 
     big_list[atomicAdd(big_count, 1)] = c;
     return;
   }
-  for (int a = s + 1; a < e; ++a) {
-    const int v = sidx[a];
-    int b = a - 1;
-    while (b >= s && sidx[b] > v) {
-      sidx[b + 1] = sidx[b];
-      --b;
-    }
-    sidx[b + 1] = v;
-  }
+  if (m <= 16)
+    sort_run_in_registers<16>(sidx + s, m);
+  else
+    sort_run_in_registers<kSmallCell>(sidx + s, m);
 }
 
 // Block-cooperative sort of long runs: a bitonic network in which every
 // compare-exchange puts the minimum at the lower slot (first step of each
 // merge compares i with i ^ (k-1)), so slots past the run end act as +inf
 // and are never touched.  Grid-stride over the queued cells.
+template <class T>
 __global__ __launch_bounds__(256) void k_sort_big_cells(const int32_t* __restrict__ off,
-                                                        int32_t* __restrict__ sidx,
+                                                        T* __restrict__ sidx,
                                                         const int32_t* __restrict__ big_list,
                                                         const int32_t* __restrict__ big_count) {
   const int count = *big_count;
@@ -455,13 +560,13 @@ __global__ __launch_bounds__(256) void k_sort_big_cells(const int32_t* __restric
     const int s = off[c], m = off[c + 1] - s;
     int P = 1;
     while (P < m) P <<= 1;
-    int32_t* a = sidx + s;
+    T* a = sidx + s;
     for (int k = 2; k <= P; k <<= 1) {
       for (int j = k >> 1; j > 0; j >>= 1) {
         for (int i = threadIdx.x; i < P; i += blockDim.x) {
           const int partner = (j == (k >> 1)) ? (i ^ (k - 1)) : (i ^ j);
           if (partner > i && partner < m) {
-            const int x = a[i], y = a[partner];
+            const T x = a[i], y = a[partner];
             if (x > y) {
               a[i] = y;
               a[partner] = x;
@@ -599,6 +704,13 @@ enum { VISIT_FULL = 0, VISIT_BUILD = 1, VISIT_LIST = 2 };
 constexpr int VISIT_STREAM = 4;        // one stream per lane; batches run on across row ends
 constexpr int VISIT_MASKED_LOADS = 8;  // with VISIT_STREAM: empty batch slots load nothing
 constexpr int VISIT_ROW_BATCH = 16;    // with VISIT_STREAM: batches stay in a row
+// Kernel template bit: walk row by row.  A scene under kStreamMinParticles
+// (about 8 waves per CU) is latency-bound - few waves hide nothing - and the
+// stream walks' bookkeeping lengthens every wave's chain: at 25,600
+// particles lambda / delta / XSPH take 51 / 48 / 58 us row by row against
+// 58 / 59 / 74 us streamed.
+constexpr int kPlainWalk = 64;
+constexpr int kStreamMinParticles = 1 << 17;
 
 struct PV {
   float4 p, v;
@@ -867,7 +979,7 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
 #pragma clang fp contract(fast)
     const float h = k.h, h2 = k.h2, r2l = L.r2_list;
     auto ld = [&](int q) { return pred_s[q]; };
-    visit_auto<MODE | VISIT_STREAM | VISIT_ROW_BATCH>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+    visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM | VISIT_ROW_BATCH)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
       const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       const float r2 = dist2_exact(dx, dy, dz);
       if (r2 < h2) {
@@ -940,7 +1052,7 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
       const bool scorr = k.scorr_k > 0.0f && k.w_q > 1e-9f;
       const float inv_wq = scorr ? 1.0f / k.w_q : 0.0f;
       auto ld = [&](int q) { return pred_in[q]; };
-      visit_auto<MODE | VISIT_STREAM>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+      visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2_exact(dx, dy, dz);
         if (r2 < h2 && r2 > 1e-12f && q != t) {
@@ -1008,7 +1120,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
                                               float4* __restrict__ pos4, float4* __restrict__ vel4,
                                               float4* __restrict__ pos_s,
                                               float4* __restrict__ pred4_out, Lists L, Scenes S,
-                                              float4* __restrict__ normal4 = nullptr) {
+                                              float4* __restrict__ normal4 = nullptr, int normal_by_slot = 0) {
   constexpr bool FUSE = (V & 4) != 0;
   constexpr bool SPLIT = (V & 16) != 0;  // dv in sorted order for k_finalize_s
   constexpr bool NORMAL = (V & 32) != 0;  // also the next substep's normal pass (k_normal)
@@ -1048,7 +1160,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
       c.r2 = dist2_exact(pi.x - c.p.x, pi.y - c.p.y, pi.z - c.p.z);
       if (c.r2 < h2 && q != t) c.v = vel_s[q];
     };
-    visit_auto<MODE | VISIT_STREAM | VISIT_MASKED_LOADS>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
+    visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM | VISIT_MASKED_LOADS)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
       const float4 pj = c.p, vj = c.v;
       const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       const float r2 = c.r2;
@@ -1079,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
     lap *= k.mass * k.visc_lap;
     const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
     const float curvature = (mag > 1e-4f) ? -lap / mag : 0.0f;
-    normal4[min(max(sidx[t], 0), n - 1)] = make_float4(nx, ny, nz, curvature);
+    normal4[normal_by_slot ? t : min(max(sidx[t], 0), n - 1)] = make_float4(nx, ny, nz, curvature);
   }
   const float f = k.visc * (k.mass * k.poly6);
   const float dvx = f * ax, dvy = f * ay, dvz = f * az;
@@ -1115,7 +1227,9 @@ __global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p, Collider 
                                                     const int32_t* __restrict__ sidx,
                                                     float4* __restrict__ delta4, float4* __restrict__ pos4,
                                                     float4* __restrict__ vel4, float4* __restrict__ pos_s,
-                                                    float4* __restrict__ pred4_out) {
+                                                    float4* __restrict__ pred4_out,
+                                                    float4* __restrict__ pos_r = nullptr,
+                                                    float4* __restrict__ vel_r = nullptr) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const float4 pi = pred_s[t], vi = vel_s[t], dv = dv_s[t];
@@ -1125,10 +1239,13 @@ __global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p, Collider 
   float px = pi.x, py = pi.y, pz = pi.z;
   if (pred4_out) pred4_out[i] = make_float4(px, py, pz, 1.0f);
   finalize_one(p, col, px, py, pz, vx, vy, vz);
-  const float4 pf = make_float4(px, py, pz, 1.0f);
-  pos4[i] = pf;
-  vel4[i] = make_float4(vx, vy, vz, 0.0f);
+  const float4 pf = make_float4(px, py, pz, 1.0f), vf = make_float4(vx, vy, vz, 0.0f);
+  if (pos4) pos4[i] = pf;
+  if (vel4) vel4[i] = vf;
   if (pos_s) pos_s[t] = pf;
+  // the state by slot for the next substep of the same call
+  if (pos_r) pos_r[t] = pf;
+  if (vel_r) vel_r[t] = vf;
 }
 
 // shaders/sph_normal.glsl:24-81 over the previous grid; pred_s.w = the
@@ -1328,6 +1445,14 @@ static int validate(const FxParams* p, const FxState* s, bool need_work) {
 // neighbour passes use the tuned block size (64, 128 or 256)
 #define NLAUNCH(kernel, n, st, ...) \
   FX_LAUNCH(kernel, dim3(grid_for(n, g_tuning.block)), dim3(g_tuning.block), st, __VA_ARGS__)
+// a full-stencil neighbour pass with the walk chosen by scene size
+#define WLAUNCH(kernel, V, n, st, ...)                    \
+  do {                                                    \
+    if ((n) >= kStreamMinParticles)                       \
+      NLAUNCH(kernel<(V)>, n, st, __VA_ARGS__);           \
+    else                                                  \
+      NLAUNCH(kernel<(V) + kPlainWalk>, n, st, __VA_ARGS__); \
+  } while (0)
 
 constexpr int32_t kInfBits = 0x7f800000;  // +inf: neighbour lists invalid
 
@@ -1356,13 +1481,25 @@ int canon_cells(const int32_t* off, int32_t* sidx, int n_cells, int32_t* big_lis
                 hipStream_t st) {
   if (n_cells <= 0) return FX_OK;
   FX_TRY(fill(big_count, 1, 0, st));
-  LAUNCH(k_canon_cells, n_cells, st, n_cells, off, sidx, big_list, big_count);
+  LAUNCH(k_canon_cells<int32_t>, n_cells, st, n_cells, off, sidx, big_list, big_count);
   return sort_queued_runs(off, sidx, big_list, big_count, st);
 }
 
 int sort_queued_runs(const int32_t* off, int32_t* data, const int32_t* queue, const int32_t* queue_count,
                      hipStream_t st) {
-  FX_LAUNCH(k_sort_big_cells, dim3(64), dim3(256), st, off, data, queue, queue_count);
+  FX_LAUNCH(k_sort_big_cells<int32_t>, dim3(kBigCellBlocks), dim3(256), st, off, data, queue, queue_count);
+  return FX_OK;
+}
+
+// The same for the (particle << 32 | previous slot) pairs of a resident
+// grid build: ordering the pairs orders the particles.
+static int canon_pairs(const int32_t* off, uint64_t* pairs, int n_cells, int32_t* big_list, int32_t* big_count,
+                       hipStream_t st) {
+  if (n_cells <= 0) return FX_OK;
+  FX_TRY(fill(big_count, 1, 0, st));
+  LAUNCH(k_canon_cells<uint64_t>, n_cells, st, n_cells, off, pairs, big_list, big_count);
+  FX_LAUNCH(k_sort_big_cells<uint64_t>, dim3(kBigCellBlocks), dim3(256), st, off, pairs, (const int32_t*)big_list,
+            (const int32_t*)big_count);
   return FX_OK;
 }
 
@@ -1409,11 +1546,56 @@ static int build_grid(const FxParams& p, FxState& s, const Ws& w, bool do_predic
   return FX_OK;
 }
 
+// Grid build of a substep whose state enters by slot of the previous
+// substep of the same fx_step call (see k_predict_resident).  The counting
+// sort carries (particle, previous slot) pairs, so each cell's run is still
+// put in ascending particle order - the same order as build_grid - and the
+// gather reads the previous slots nearly in sequence instead of gathering
+// the canonical arrays.  The reference-format outputs (keys, sorted keys and
+// indices, cell ranges, normals) go to the caller on the call's last
+// substep only.
+static int build_grid_resident(const FxParams& p, FxState& s, const Ws& w, int target, bool last_in_call,
+                               hipStream_t st) {
+  const int n = p.n, nc = p.n_cells;
+  Grid g = make_grid(p);
+  FX_TRY(fill(w.cnt, nc + 1, 0, st));
+  float4* pred_r = w.pred_s[target ^ 1];  // free: the previous substep is finalized
+  const bool cohesion = p.surface_tension > 0.0f;
+  LAUNCH(k_predict_resident, n, st, p, g, (const float4*)w.pos_r, (const float4*)w.vel_r,
+         cohesion ? (const float4*)w.normal_next : (const float4*)nullptr, pred_r, w.key, w.cnt, w.rank,
+         (const int32_t*)w.sidx, (cohesion && last_in_call) ? (float4*)s.normal : (float4*)nullptr);
+  FX_TRY(exclusive_scan_i32(w.cnt, w.off, nc, w.partial, st));
+  if (last_in_call && s.cell_start && s.cell_end) LAUNCH(k_cell_ranges, nc, st, nc, w.off, s.cell_start, s.cell_end);
+  LAUNCH(k_scatter_pairs, n, st, n, (const int32_t*)w.key, (const int32_t*)w.rank, (const int32_t*)w.off,
+         (const int32_t*)w.sidx, w.pair64);
+  // big-cell queue in the rank array (dead after the scatter)
+  FX_TRY(canon_pairs(w.off, w.pair64, nc, w.rank, w.flags, st));
+  LAUNCH(k_gather_pairs, n, st, n, (const uint64_t*)w.pair64, (const int32_t*)w.key, (const float4*)pred_r,
+         (const float4*)w.pos_r, w.pred_s[target], w.pos_s, w.dl_s, last_in_call ? s.sorted_key : nullptr, w.skey,
+         w.sidx, last_in_call ? s.sorted_idx : nullptr, last_in_call ? s.key : nullptr);
+  forget_normals(s.work);
+  FX_TRY(fill(w.flags + 2, 1, kInfBits, st));
+  FX_TRY(check_launch("build_grid_resident"));
+  mark_grid(s.work, n, nc);
+  return FX_OK;
+}
+
 // last: this substep's state is what the caller sees next.  Earlier
 // substeps of one fx_step call leave out the outputs the next substep
 // overwrites before anything reads them: the canonical prediction, lambda and
 // correction, and the sorted final positions (read by the surface).
-static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, bool last_in_call = true) {
+// Whether the substeps of one fx_step call can hand their state on by slot
+// (build_grid_resident): not with neighbour lists (their XSPH pass writes the
+// canonical state itself) and, with cohesion, only when the XSPH pass
+// computes the next normals (iterations > 0).
+static bool resident_ok(const FxParams& p, const FxState& s) {
+  return p.n > 0 && g_tuning.skin <= 0.0f && (p.surface_tension <= 0.0f || (s.normal && p.iterations > 0));
+}
+
+// resident_in: the state enters by slot from the previous substep of the
+// call; resident_out: it leaves that way for the next one.
+static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, bool last_in_call = true,
+                   bool resident_in = false, bool resident_out = false) {
   const int n = p.n;
   Grid g = make_grid(p);
   Consts k = make_consts(p);
@@ -1421,8 +1603,9 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
   const bool lists = g_tuning.skin > 0.0f;
   Lists L = make_lists(w, k, lists);
   // sph_normal runs first, over the previous substep's grid and state
-  // (shaders/sph_normal.glsl:15-21); its output feeds sph_predict.
-  if (p.surface_tension > 0.0f) {
+  // (shaders/sph_normal.glsl:15-21); its output feeds sph_predict.  A
+  // resident substep takes the normals by slot from the previous XSPH pass.
+  if (p.surface_tension > 0.0f && !resident_in) {
     if (!s.normal) return fail(FX_EINVAL, "surface_tension > 0 needs a normal buffer");
     if (!grid_ready(s.work, n, p.n_cells))
       return fail(FX_EINVAL, "surface tension reads the previous grid: call fx_load_state first");
@@ -1440,7 +1623,9 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
   }
   const int iters = p.iterations;
   int cur = iters & 1;  // the last iteration writes buffer 0
-  {
+  if (resident_in) {
+    FX_TRY(build_grid_resident(p, s, w, cur, last_in_call, st));
+  } else {
     // the reference-format grid (keys, sorted keys and indices, cell ranges)
     // is published for the caller only by the call's last substep
     FxState sg = s;
@@ -1456,7 +1641,7 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
     const bool last = it == iters - 1;
     float4* lam_out = (last && last_in_call && s.lambda) ? (float4*)s.lambda : (float4*)nullptr;
     if (!lists)
-      NLAUNCH(k_lambda<VISIT_FULL>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
+      WLAUNCH(k_lambda, VISIT_FULL, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else if (it == 0)
       NLAUNCH(k_lambda<VISIT_BUILD>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else
@@ -1466,7 +1651,7 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
               w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
               (float4*)nullptr, L, (const int32_t*)w.skey, Scenes{nullptr, 0});
     else
-      NLAUNCH(k_delta<VISIT_FULL>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
+      WLAUNCH(k_delta, VISIT_FULL, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
               w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
               (float4*)nullptr, L, (const int32_t*)nullptr, Scenes{nullptr, 0});
     cur ^= 1;
@@ -1484,18 +1669,26 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
     // cohesion on, the same traversal yields the next substep's normals
     const bool fuse_normals = p.surface_tension > 0.0f && s.normal && iters > 0;
     if (fuse_normals) {
-      NLAUNCH(k_xsph<32 + 16 + VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
+      // resident: by slot for the next substep; last of the call: for the
+      // next call; otherwise straight into the caller's buffer
+      float4* nrm = (resident_out || last_in_call) ? w.normal_next : (float4*)s.normal;
+      WLAUNCH(k_xsph, 32 + 16 + VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
               (const float4*)w.vel_s, w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr,
-              (float4*)nullptr, (float4*)nullptr, L, Scenes{nullptr, 0},
-              last_in_call ? w.normal_next : (float4*)s.normal);
+              (float4*)nullptr, (float4*)nullptr, L, Scenes{nullptr, 0}, nrm, resident_out ? 1 : 0);
     } else {
-      NLAUNCH(k_xsph<16 + VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+      WLAUNCH(k_xsph, 16 + VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
               w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, (float4*)nullptr,
               L, Scenes{nullptr, 0});
     }
-    LAUNCH(k_finalize_s, n, st, n, p, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
-           (const float4*)w.pred_s[1], w.sidx, delta_out, (float4*)s.pos, (float4*)s.vel, pos_s_out, pred_out);
-    if (fuse_normals) mark_normals(s.work, s.normal, p, !last_in_call);
+    if (resident_out)
+      LAUNCH(k_finalize_s, n, st, n, p, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+             (const float4*)w.pred_s[1], w.sidx, (float4*)nullptr, (float4*)nullptr, (float4*)nullptr,
+             (float4*)nullptr, (float4*)nullptr, w.pos_r, w.vel_r);
+    else
+      LAUNCH(k_finalize_s, n, st, n, p, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+             (const float4*)w.pred_s[1], w.sidx, delta_out, (float4*)s.pos, (float4*)s.vel, pos_s_out, pred_out,
+             (float4*)nullptr, (float4*)nullptr);
+    if (fuse_normals && !resident_out) mark_normals(s.work, s.normal, p, !last_in_call);
   }
   return check_launch("substep");
 }
@@ -1587,7 +1780,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
         LAUNCH(k_gather, n, st, n, w.sidx, (const int32_t*)nullptr, (const float4*)s.pred, (const float4*)nullptr,
                (const float4*)nullptr, w.pred_s[0], (float4*)nullptr, (float2*)nullptr,
                (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
-        NLAUNCH(k_lambda<VISIT_FULL>, n, st, n, g, k, w.pred_s[0], w.off, w.dl_s, w.sidx, (float4*)s.lambda, L0, Scenes{nullptr, 0});
+        WLAUNCH(k_lambda, VISIT_FULL, n, st, n, g, k, w.pred_s[0], w.off, w.dl_s, w.sidx, (float4*)s.lambda, L0, Scenes{nullptr, 0});
       }
       break;
     case FX_PASS_DELTA:
@@ -1601,7 +1794,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
                (const float4*)s.lambda, w.pred_s[0], (float4*)nullptr, w.dl_s, (int32_t*)nullptr,
                (int32_t*)nullptr, (int32_t*)nullptr);
         LAUNCH(k_w_from_lambda, n, st, n, w.pred_s[0], (const float2*)w.dl_s);
-        NLAUNCH(k_delta<VISIT_FULL>, n, st, n, g, k, (const float4*)w.pred_s[0], (const float2*)w.dl_s, w.off,
+        WLAUNCH(k_delta, VISIT_FULL, n, st, n, g, k, (const float4*)w.pred_s[0], (const float2*)w.dl_s, w.off,
                 (float4*)nullptr, (const float4*)nullptr, (float4*)nullptr, w.sidx, (float4*)s.delta, L0,
                 (const int32_t*)nullptr, Scenes{nullptr, 0});
       }
@@ -1626,7 +1819,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
       if (n > 0) {
         LAUNCH(k_gather_xsph, n, st, n, w.sidx, (const float4*)s.pred, (const float4*)s.vel,
                (const float4*)s.lambda, w.pred_s[0], w.vel_s);
-        NLAUNCH(k_xsph<VISIT_FULL>, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
+        WLAUNCH(k_xsph, VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
                 (const float4*)w.vel_s, w.off, w.sidx, (float4*)s.delta, (float4*)nullptr,
                 (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, L0, Scenes{nullptr, 0});
       }
@@ -1874,7 +2067,11 @@ int fx_step(const FxParams* p, FxState* s, int substeps, void* stream) {
   if (p->n > 0 && (!s->pos || !s->vel)) return fail(FX_EINVAL, "substep needs pos and vel");
   if (p->n > 0 && !s->pred) return fail(FX_EINVAL, "substep needs a pred buffer");
   Ws w = bind_ws(workspace_layout(*p), s->work);
-  for (int i = 0; i < substeps; ++i) FX_TRY(substep(*p, *s, w, (hipStream_t)stream, i == substeps - 1));
+  const bool resident = resident_ok(*p, *s);
+  for (int i = 0; i < substeps; ++i) {
+    const bool last = i == substeps - 1;
+    FX_TRY(substep(*p, *s, w, (hipStream_t)stream, last, resident && i > 0, resident && !last));
+  }
   return FX_OK;
 }
 
