@@ -1446,12 +1446,13 @@ static int validate(const FxParams* p, const FxState* s, bool need_work) {
 #define NLAUNCH(kernel, n, st, ...) \
   FX_LAUNCH(kernel, dim3(grid_for(n, g_tuning.block)), dim3(g_tuning.block), st, __VA_ARGS__)
 // a full-stencil neighbour pass with the walk chosen by scene size
-#define WLAUNCH(kernel, V, n, st, ...)                    \
-  do {                                                    \
-    if ((n) >= kStreamMinParticles)                       \
-      NLAUNCH(kernel<(V)>, n, st, __VA_ARGS__);           \
-    else                                                  \
-      NLAUNCH(kernel<(V) + kPlainWalk>, n, st, __VA_ARGS__); \
+// (V unparenthesised: the launch's profile label is the kernel text)
+#define WLAUNCH(kernel, V, n, st, ...)                     \
+  do {                                                     \
+    if ((n) >= kStreamMinParticles)                        \
+      NLAUNCH(kernel<V>, n, st, __VA_ARGS__);              \
+    else                                                   \
+      NLAUNCH(kernel<V + kPlainWalk>, n, st, __VA_ARGS__); \
   } while (0)
 
 constexpr int32_t kInfBits = 0x7f800000;  // +inf: neighbour lists invalid
