@@ -313,34 +313,55 @@ __device__ __forceinline__ int wave_block_scan(int v, int* lds, int* total) {
   return v + add;
 }
 
+// Lattice coordinates of sample q.  fx_mc_extract caps the lattice at 2^28
+// samples, so 32-bit arithmetic holds (two 32-bit divisions instead of three
+// 64-bit ones, which dominated these kernels' instruction count).
+__device__ __forceinline__ void mc_ijk(const McLattice& L, int q, int& i, int& j, int& k) {
+  const unsigned row = (unsigned)q / (unsigned)L.nx;
+  const unsigned kk = row / (unsigned)L.ny;
+  i = q - (int)(row * (unsigned)L.nx);
+  j = (int)(row - kk * (unsigned)L.ny);
+  k = (int)kk;
+}
+
 // Per sample: the sign-change flags of its +x / +y / +z lattice edges (the
 // vertices it owns) and, if it is a cube's lower corner, the cube's case.
+// Each of the cube's eight corner values is loaded once; corner c of the
+// case index sits at kCorners[c] (solver/marching_cubes.py:25-34).
 __global__ __launch_bounds__(kMcBlock) void k_mc_classify(McLattice L, const float* __restrict__ f,
                                                           uint16_t* __restrict__ code,
                                                           int32_t* __restrict__ bv,
                                                           int32_t* __restrict__ bt,
                                                           uint8_t* __restrict__ cases) {
   __shared__ int lds[4];
-  const long long q = (long long)blockIdx.x * kMcBlock + threadIdx.x;
+  const int q = blockIdx.x * kMcBlock + threadIdx.x;
   int nv = 0, nt = 0;
-  if (q < L.S) {
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
-    const long long nxy = (long long)nx * ny;
-    const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / nxy);
-    const bool in0 = f[q] >= L.iso;
+  if (q < (int)L.S) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz, nxy = nx * ny;
+    int i, j, k;
+    mc_ijk(L, q, i, j, k);
+    const bool hx = i + 1 < nx, hy = j + 1 < ny, hz = k + 1 < nz;
+    // v[x + 2 y + 4 z] = the field at corner (x, y, z) of the cube at q
+    bool v[8];
+    v[0] = f[q] >= L.iso;
+    v[1] = hx && f[q + 1] >= L.iso;
+    v[2] = hy && f[q + nx] >= L.iso;
+    v[4] = hz && f[q + nxy] >= L.iso;
     int flags = 0;
-    if (i + 1 < nx && ((f[q + 1] >= L.iso) != in0)) flags |= 1;
-    if (j + 1 < ny && ((f[q + nx] >= L.iso) != in0)) flags |= 2;
-    if (k + 1 < nz && ((f[q + nxy] >= L.iso) != in0)) flags |= 4;
+    if (hx && v[1] != v[0]) flags |= 1;
+    if (hy && v[2] != v[0]) flags |= 2;
+    if (hz && v[4] != v[0]) flags |= 4;
     int cs = 0;
-    if (i + 1 < nx && j + 1 < ny && k + 1 < nz) {
+    if (hx && hy && hz) {
+      v[3] = f[q + 1 + nx] >= L.iso;
+      v[5] = f[q + 1 + nxy] >= L.iso;
+      v[6] = f[q + nx + nxy] >= L.iso;
+      v[7] = f[q + 1 + nx + nxy] >= L.iso;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const long long o = kMc.corner[c][0] + (long long)kMc.corner[c][1] * nx + kMc.corner[c][2] * nxy;
-        if (f[q + o] >= L.iso) cs |= 1 << c;
-      }
+      for (int c = 0; c < 8; ++c)
+        if (v[kCorners[c][0] + 2 * kCorners[c][1] + 4 * kCorners[c][2]]) cs |= 1 << c;
       nt = kMc.ntri[cs];
-      if (cases) cases[((long long)k * (ny - 1) + j) * (nx - 1) + i] = (uint8_t)cs;
+      if (cases) cases[(k * (ny - 1) + j) * (nx - 1) + i] = (uint8_t)cs;
     }
     nv = __popc(flags);
     code[q] = (uint16_t)(flags | (cs << 3));
@@ -367,9 +388,9 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(McLattice L, const flo
                                                           float* __restrict__ verts,
                                                           long long vcap) {
   __shared__ int lds[4];
-  const long long q = (long long)blockIdx.x * kMcBlock + threadIdx.x;
+  const int q = blockIdx.x * kMcBlock + threadIdx.x;
   int flags = 0, cs = 0;
-  if (q < L.S) {
+  if (q < (int)L.S) {
     const int c = code[q];
     flags = c & 7;
     cs = c >> 3;
@@ -378,19 +399,19 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_vertices(McLattice L, const flo
   int tv, tt;
   const int iv = wave_block_scan(nv, lds, &tv) - nv + bv_off[blockIdx.x];
   const int it = wave_block_scan(nt, lds, &tt) - nt + bt_off[blockIdx.x];
-  if (q >= L.S) return;
+  if (q >= (int)L.S) return;
   vbase[q] = (uint32_t)iv;
   tbase[q] = (uint32_t)it;
   if (!flags) return;
-  const int nx = L.nx, ny = L.ny;
-  const long long nxy = (long long)nx * ny;
-  const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / nxy);
+  const int nx = L.nx, nxy = L.nx * L.ny;
+  int i, j, k;
+  mc_ijk(L, q, i, j, k);
   const double lo_v = (double)f[q];
   const double iso = (double)L.iso;
   int r = 0;
   for (int axis = 0; axis < 3; ++axis) {
     if (!(flags & (1 << axis))) continue;
-    const long long stride = axis == 0 ? 1 : (axis == 1 ? nx : nxy);
+    const int stride = axis == 0 ? 1 : (axis == 1 ? nx : nxy);
     const double hi_v = (double)f[q + stride];
     const double delta = hi_v - lo_v;
     const double t = delta == 0.0 ? 0.5 : (iso - lo_v) / delta;
@@ -413,19 +434,19 @@ __global__ __launch_bounds__(kMcBlock) void k_mc_triangles(McLattice L,
                                                            const uint32_t* __restrict__ vbase,
                                                            const uint32_t* __restrict__ tbase,
                                                            int32_t* __restrict__ tris, long long tcap) {
-  const long long q = (long long)blockIdx.x * kMcBlock + threadIdx.x;
-  if (q >= L.S) return;
+  const int q = blockIdx.x * kMcBlock + threadIdx.x;
+  if (q >= (int)L.S) return;
   const int cs = code[q] >> 3;
   const int nt = kMc.ntri[cs];
   if (!nt) return;
-  const long long nx = L.nx, nxy = (long long)L.nx * L.ny;
+  const int nx = L.nx, nxy = L.nx * L.ny;
   const long long base = tbase[q];
   for (int m = 0; m < nt; ++m) {
     int idx[3];
     for (int e3 = 0; e3 < 3; ++e3) {
       const int e = kMc.tri[cs][3 * m + e3];
       const int lc = kMc.walk[e][0], axis = kMc.walk[e][2];
-      const long long q2 = q + kMc.corner[lc][0] + kMc.corner[lc][1] * nx + kMc.corner[lc][2] * nxy;
+      const int q2 = q + kMc.corner[lc][0] + kMc.corner[lc][1] * nx + kMc.corner[lc][2] * nxy;
       const int fl = code[q2] & 7;
       idx[e3] = (int)vbase[q2] + __popc(fl & ((1 << axis) - 1));
     }
