@@ -66,6 +66,40 @@ def test_cell_beyond_max_cell_scan(fx, oracle):
     assert e_gpu64 <= max(2 * e_orc64, TOL) and e <= 1e-4
 
 
+def test_splat_dense_cells(fx, oracle):
+    """The splat over a lattice plus a 3000-particle clump spread over 2x2x2
+    cells (the tiles around it stage more particles than their LDS holds and
+    take the per-row path) and a 700-particle cell elsewhere (both paths stop
+    each cell's scan after 512 slots): bit-exact against the oracle."""
+    import ctypes
+
+    import torch
+
+    rng = np.random.default_rng(5)
+    cfg, _, _ = _params(fx, 0)
+    lattice = fx.seed_positions(cfg)
+    clump = rng.uniform([-0.19, -0.19, 0.21], [-0.01, -0.01, 0.39], (3000, 3))
+    cell = rng.uniform([0.21, 0.21, 0.61], [0.29, 0.29, 0.69], (700, 3))
+    extra = np.concatenate([np.concatenate([clump, cell]), np.ones((3700, 1))], 1).astype(np.float32)
+    pos = np.concatenate([lattice, extra])
+    cfg, p, dt = _params(fx, len(pos))
+    keys, idx, start, end = oracle.build_grid(p, pos, "stable")
+    counts = (end - start)[start >= 0]
+    assert counts.max() > 512
+    n = len(pos)
+    g = GpuState(fx, p, pos=pos, pred=pos)
+    g.run("build_grid")
+    for resolution in (48, 96):
+        surf = fx.resolve(cfg, resolution, 0.5)
+        ref = oracle.splat(p, surf.struct(), pos, idx[:n], start, end, None)
+        field = torch.zeros(surf.sample_count, dtype=torch.float32, device="cuda")
+        fx._lib.check(fx._lib.lib().fx_splat(ctypes.byref(p), ctypes.byref(surf.struct()), ctypes.byref(g.state),
+                                             ctypes.c_void_p(field.data_ptr()), g.stream()), "fx_splat")
+        out = field.cpu().numpy()
+        print(f"dense splat {surf.dims}: max field {ref.max():.2f}, max |diff| {np.abs(out - ref).max():.3e}")
+        assert np.array_equal(out, ref.astype(np.float32))
+
+
 def test_out_of_domain_predictions_clamp_into_edge_cells(fx, oracle):
     rng = np.random.default_rng(12)
     n = 4000

@@ -279,9 +279,12 @@ def test_substep_deterministic(fx, scene):
         assert np.array_equal(a, b)
 
 
-def test_splat(fx, oracle, scene):
+@pytest.mark.parametrize("resolution", [16, 64, 128])
+def test_splat(fx, oracle, scene, resolution):
+    """The splat field, bit-exact: resolution 16 runs one lattice row per
+    thread (its lattice is coarser than the grid), 64 and 128 the LDS tiles."""
     cfg, p, st, col = scene["cfg"], scene["p"], scene["st"], scene["col"]
-    surf = fx.resolve(cfg, 64, 0.5)
+    surf = fx.resolve(cfg, resolution, 0.5)
     keys, idx, start, end = _stable_grid(oracle, p, st.pred, scene["n"])
     ref = oracle.splat(p, surf.struct(), st.pos, idx, start, end, col)
     import ctypes
