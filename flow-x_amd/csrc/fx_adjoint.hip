@@ -272,6 +272,31 @@ __global__ void k_adj_cap(int n, long long cap, int32_t* __restrict__ noff, int3
   }
 }
 
+// ---- list walks ---------------------------------------------------------------
+// f(j, p[j]) for the entries j of list L[0, m) in order, kAdjBatch at a time:
+// the batch's indices, then its positions, are all loaded before any entry
+// is used (one entry at a time, every position load waits on its own index
+// load).  Empty slots of the last batch re-load its first entry so the loads
+// stay unconditional.  The order of the calls, and so of every sum, is the
+// list's.
+constexpr int kAdjBatch = 4;
+
+template <int B, class T, class F>
+__device__ __forceinline__ void walk_list(const int32_t* __restrict__ L, int m, const V3<T>* __restrict__ P,
+                                          F&& f) {
+  for (int k = 0; k < m; k += B) {
+    int j[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) j[u] = L[min(k + u, m - 1)];
+    V3<T> q[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) q[u] = P[j[u]];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (k + u < m) f(j[u], q[u]);
+  }
+}
+
 // ---- forward passes ---------------------------------------------------------
 
 // shaders/sph_predict.glsl:23-57 (no cohesion); also stores the clamped
@@ -308,8 +333,8 @@ __global__ void k_adj_lambda(int n, AdjC<T> c, const V3<T>* __restrict__ p, cons
   V3<T> gs = v3(T(0), T(0), T(0));
   T gn2 = T(0);
   const int32_t* L = nbr + noff[i];
-  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
-    const V3<T> d = pi - p[L[k]];
+  walk_list<kAdjBatch>(L, noff[i + 1] - noff[i], p, [&](const int j, const V3<T>& pj) {
+    const V3<T> d = pi - pj;
     const T r2 = dot(d, d);
     if (r2 < c.h2) {
       const T w = c.h2 - r2;
@@ -321,7 +346,7 @@ __global__ void k_adj_lambda(int n, AdjC<T> c, const V3<T>* __restrict__ p, cons
       gs = gs + gij;
       gn2 = gn2 + dot(gij, gij);
     }
-  }
+  });
   dens = c.mass * dens;
   const T rr = dens > T(1e-6) ? dens : T(1e-6);
   const T C = rr / c.rho0 - T(1);
@@ -344,15 +369,14 @@ __global__ void k_adj_delta(int n, AdjC<T> c, const V3<T>* __restrict__ p, const
   const T li = lam[i];
   V3<T> acc = v3(T(0), T(0), T(0));
   const int32_t* L = nbr + noff[i];
-  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
-    const int j = L[k];
-    const V3<T> d = pi - p[j];
+  walk_list<kAdjBatch>(L, noff[i + 1] - noff[i], p, [&](const int j, const V3<T>& pj) {
+    const V3<T> d = pi - pj;
     const T r2 = dot(d, d);
     V3<T> ph;
     T f, fpr;
     phi_and_jac(c, d, r2, ph, f, fpr);
     acc = acc + (li + lam[j]) * ph;
-  }
+  });
   pn[i] = pi + c.K * acc;
 }
 
@@ -494,15 +518,14 @@ __global__ void k_adj_xsph_final(int n, AdjC<T> c, const V3<T>* __restrict__ pI,
   const V3<T> pi = pI[i], vi = v2[i];
   V3<T> acc = v3(T(0), T(0), T(0));
   const int32_t* L = nbr + noff[i];
-  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
-    const int j = L[k];
-    const V3<T> d = pi - pI[j];
+  walk_list<kAdjBatch>(L, noff[i + 1] - noff[i], pI, [&](const int j, const V3<T>& pj) {
+    const V3<T> d = pi - pj;
     const T r2 = dot(d, d);
     if (r2 < c.h2) {
       const T w = c.h2 - r2;
       acc = acc + ((c.mass / rho[j]) * (c.poly6 * w * w * w)) * (v2[j] - vi);
     }
-  }
+  });
   V3<T> p = pi, v = vi + c.visc * acc;
   FinalRec<T> rec;
   adj_finalize(c, i, p, v, rec);
@@ -526,15 +549,14 @@ __global__ void k_adj_final_bwd(int n, AdjC<T> c, const V3<T>* __restrict__ pI, 
   const V3<T> pi = pI[i], vi = v2[i];
   V3<T> acc = v3(T(0), T(0), T(0));
   const int32_t* L = nbr + noff[i];
-  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
-    const int j = L[k];
-    const V3<T> d = pi - pI[j];
+  walk_list<kAdjBatch>(L, noff[i + 1] - noff[i], pI, [&](const int j, const V3<T>& pj) {
+    const V3<T> d = pi - pj;
     const T r2 = dot(d, d);
     if (r2 < c.h2) {
       const T w = c.h2 - r2;
       acc = acc + ((c.mass / rho[j]) * (c.poly6 * w * w * w)) * (v2[j] - vi);
     }
-  }
+  });
   const V3<T> v3in = vi + c.visc * acc;
   V3<T> p = pi, v = v3in;
   FinalRec<T> rec;
@@ -598,11 +620,10 @@ __global__ void k_adj_xsph_bwd(int n, AdjC<T> c, const V3<T>* __restrict__ pI, c
   V3<T> gp = v3(T(0), T(0), T(0)), gv = gi;
   T gr = T(0);
   const int32_t* L = nbr + noff[i];
-  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
-    const int j = L[k];
-    const V3<T> d = pi - pI[j];
+  walk_list<kAdjBatch>(L, noff[i + 1] - noff[i], pI, [&](const int j, const V3<T>& pj) {
+    const V3<T> d = pi - pj;
     const T r2 = dot(d, d);
-    if (!(r2 < c.h2)) continue;
+    if (!(r2 < c.h2)) return;
     const V3<T> vj = v2[j], gj = gv3[j];
     const T rj = rho[j];
     const T w = c.h2 - r2;
@@ -613,7 +634,7 @@ __global__ void k_adj_xsph_bwd(int n, AdjC<T> c, const V3<T>* __restrict__ pI, c
     const T s = c.visc * ((c.mass / rj) * dot(vj - vi, gi) + (c.mass / ri) * dot(vi - vj, gj));
     gp = gp + s * gradW;
     gr = gr - c.visc * (c.mass / (ri * ri)) * W * dot(vi - vj, gj);
-  }
+  });
   gpI[i] = gpI[i] + gp;
   gv2[i] = gv;
   grho[i] = gr;
@@ -644,14 +665,13 @@ __global__ void k_adj_iter_bwd1(int n, AdjC<T> c, const V3<T>* __restrict__ p, c
   const V3<T> pi = p[i], ai = a[i];
   T gl = T(0);
   const int32_t* L = nbr + noff[i];
-  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
-    const int j = L[k];
-    const V3<T> d = pi - p[j];
+  walk_list<kAdjBatch>(L, noff[i + 1] - noff[i], p, [&](const int j, const V3<T>& pj) {
+    const V3<T> d = pi - pj;
     V3<T> ph;
     T f, fpr;
     phi_and_jac(c, d, dot(d, d), ph, f, fpr);
     gl = gl + dot(ai - a[j], ph);
-  }
+  });
   gl = c.K * gl;
   const T D = Dv[i];
   const T C = rho[i] / c.rho0 - T(1);
@@ -675,11 +695,10 @@ __global__ void k_adj_iter_bwd2(int n, AdjC<T> c, const V3<T>* __restrict__ p, c
   const T li = lam[i], gSi = gS[i], gri = grho[i];
   V3<T> acc = ai;
   const int32_t* L = nbr + noff[i];
-  for (int k = 0, m = noff[i + 1] - noff[i]; k < m; ++k) {
-    const int j = L[k];
-    const V3<T> d = pi - p[j];
+  walk_list<kAdjBatch>(L, noff[i + 1] - noff[i], p, [&](const int j, const V3<T>& pj) {
+    const V3<T> d = pi - pj;
     const T r2 = dot(d, d);
-    if (!(r2 < c.h2)) continue;
+    if (!(r2 < c.h2)) return;
     V3<T> ph;
     T f, fpr;
     phi_and_jac(c, d, r2, ph, f, fpr);
@@ -691,7 +710,7 @@ __global__ void k_adj_iter_bwd2(int n, AdjC<T> c, const V3<T>* __restrict__ p, c
     // density: m dW/dp_i (grho_i + grho_j)
     const T w = c.h2 - r2;
     acc = acc + (c.mass * (gri + grho[j]) * T(-6) * c.poly6 * w * w) * d;
-  }
+  });
   gp_out[i] = acc;
 }
 
