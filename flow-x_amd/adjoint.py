@@ -173,18 +173,23 @@ class Splat(torch.autograd.Function):
 
 
 def rollout(solver, x0, v0, gravity, translation, substeps, decisions=None, mode=0, check_overflow=True):
-    """`substeps` differentiable substeps.  With decisions = [] and mode 1 the
-    per-substep records (branches and neighbour lists) are appended to the
-    list; mode 2 replays a list recorded earlier.  Raises if a neighbour list
-    overflowed its capacity, since the gradient of a truncated list is not
-    that of the substep."""
+    """`substeps` differentiable substeps.  With a list `decisions` and mode 1
+    the per-substep records (branches and neighbour lists) are written into
+    the list: entry k is reused when present (records allocated ahead, e.g.
+    with solver.new_decisions(), keep allocation out of the loop), else a new
+    record is appended.  Mode 2 replays a list recorded earlier.  Raises if a
+    neighbour list overflowed its capacity, since the gradient of a truncated
+    list is not that of the substep."""
     x, v = x0, v0
     for k in range(substeps):
         dec = None
         if decisions is not None:
             if mode == 1:
-                dec = solver.new_decisions()
-                decisions.append(dec)
+                if k < len(decisions):
+                    dec = decisions[k]
+                else:
+                    dec = solver.new_decisions()
+                    decisions.append(dec)
             elif mode == 2:
                 dec = decisions[k]
         x, v = Substep.apply(x, v, gravity, translation, solver, dec, mode)

@@ -235,6 +235,29 @@ def test_adjoint_recorded_lists_are_sorted_stencil_pairs(fx):
             assert np.array_equal(got, np.flatnonzero(want[i - a])), i
 
 
+def test_rollout_reuses_preallocated_records(fx):
+    """A recording rollout into records allocated ahead equals one that
+    appends new records: same positions, same records."""
+    import torch
+
+    from flowx_amd.adjoint import rollout
+
+    cfg, d, x0 = _cluster_scene(fx, (12, 12, 6), 50)
+    v0 = torch.zeros_like(x0)
+    g = torch.tensor(-9.81, dtype=torch.float64, device="cuda")
+    tr = torch.zeros(3, dtype=torch.float64, device="cuda")
+    appended = []
+    with torch.no_grad():
+        xa, va = rollout(d, x0, v0, g, tr, 3, appended, 1)
+        ahead = [d.new_decisions().fill_(-7) for _ in range(3)]
+        xb, vb = rollout(d, x0, v0, g, tr, 3, ahead, 1)
+    assert len(ahead) == 3 and torch.equal(xa, xb) and torch.equal(va, vb)
+    n = len(x0)
+    for ra, rb in zip(appended, ahead):
+        used = 3 * n + 1 + int(ra[3 * n].item())  # branches, offsets, then the lists
+        assert torch.equal(ra[:used], rb[:used])
+
+
 def test_adjoint_list_overflow_raises(fx):
     """More pairs than n * 192 in total: the capped lists are flagged and the
     rollout raises instead of returning a wrong gradient."""
