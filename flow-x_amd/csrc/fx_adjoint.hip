@@ -181,22 +181,33 @@ __global__ void k_adj_lists(int n, Grid g, AdjC<T> c, const V3<T>* __restrict__ 
     L = nbr + noff[i];
     room = noff[i + 1] - noff[i];
   }
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int x = bx + dx, y = by + dy, z = bz + dz;
-        if (x < 0 || y < 0 || z < 0 || x >= g.cx || y >= g.cy || z >= g.cz) continue;
-        const int cidx = cell_index(g, x, y, z);
-        for (int s = off[cidx]; s < off[cidx + 1]; ++s) {
-          const int j = sidx[s];
-          if (j == i) continue;
-          const V3<T> d = pi - p[j];
-          if (dot(d, d) < c.list_r2) {
-            if (FILL && m < room) L[m] = j;
-            ++m;
-          }
+  // the 27 cells as 9 runs of 3 adjacent cells (x-1..x+1 are contiguous in
+  // sidx), each walked 4 slots at a time: the slots' particle indices, then
+  // their positions, are loaded before any is tested
+  const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
+  for (int r = 0; r < 9; ++r) {
+    const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
+    if (y < 0 || z < 0 || y >= g.cy || z >= g.cz) continue;
+    const int c0 = cell_index(g, x0, y, z);
+    const int e = off[c0 + (x1 - x0) + 1];
+    for (int q = off[c0]; q < e; q += 4) {
+      int j[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) j[u] = sidx[min(q + u, e - 1)];
+      V3<T> pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pj[u] = p[j[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (q + u >= e || j[u] == i) continue;
+        const V3<T> d = pi - pj[u];
+        if (dot(d, d) < c.list_r2) {
+          if (FILL && m < room) L[m] = j[u];
+          ++m;
         }
       }
+    }
+  }
   if (!FILL) count[i] = m;
 }
 
