@@ -166,17 +166,17 @@ struct NoRefine {
   __device__ __forceinline__ void operator()(int, D&) const {}
 };
 
-template <class LD, class F, class RF = NoRefine>
+template <int B = FX_BATCH, class LD, class F, class RF = NoRefine>
 __device__ __forceinline__ void scan_run(int s, int e, LD&& ld, F&& f, RF&& refine = RF()) {
   int q = s;
-  for (; q + FX_BATCH <= e; q += FX_BATCH) {
-    decltype(ld(q)) d[FX_BATCH];
+  for (; q + B <= e; q += B) {
+    decltype(ld(q)) d[B];
 #pragma unroll
-    for (int u = 0; u < FX_BATCH; ++u) d[u] = ld(q + u);
+    for (int u = 0; u < B; ++u) d[u] = ld(q + u);
 #pragma unroll
-    for (int u = 0; u < FX_BATCH; ++u) refine(q + u, d[u]);
+    for (int u = 0; u < B; ++u) refine(q + u, d[u]);
 #pragma unroll
-    for (int u = 0; u < FX_BATCH; ++u) f(q + u, d[u]);
+    for (int u = 0; u < B; ++u) f(q + u, d[u]);
   }
   for (; q < e; ++q) {
     auto d = ld(q);
