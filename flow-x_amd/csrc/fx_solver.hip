@@ -835,10 +835,12 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
     }
   }
   if (M == VISIT_FULL) {
-    // row by row (small scenes, latency-bound: deeper batches shorten each
+    // row by row: small scenes (latency-bound: deeper batches shorten each
     // lane's chain of loads; dam break at 25,600 particles with 4 / 8 / 12
-    // loads in flight: 57.0 / 55.5 / 57.4 ms)
-    constexpr int kPlainBatch = 8;
+    // loads in flight: 57.0 / 55.5 / 57.4 ms), and the stream walks' lanes
+    // with a row past max_scan, which keep FX_BATCH - this path's registers
+    // count against the whole kernel (8 would take XSPH from 73 to 107 VGPRs)
+    constexpr int kPlainBatch = (MODE & VISIT_STREAM) ? FX_BATCH : 8;
     // the next row's run bounds are loaded while the current row is scanned
     auto row = [&](int r, int& c0, int& s, int& e) {
       const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
