@@ -426,32 +426,46 @@ __global__ void k_scatter(int n, const int32_t* __restrict__ key, const int32_t*
 }
 
 // ---- resident grid build (substeps after the first of one fx_step call) ----
-// The state enters in the previous substep's sorted order: slot t holds
-// particle pid[t], its final position pos_r[t] and velocity vel_r[t], and
-// the normal this substep uses nrm_r[t].  Predict runs per slot; the
-// predicted velocity is not stored (the velocity pass recomputes it from the
-// positions, shaders/sph_velocity.glsl:12-24).  nrm_canon (the call's last
-// substep only) receives the canonical normals the caller's normal buffer
-// holds after the substep.
-__global__ __launch_bounds__(256) void k_predict_resident(FxParams p, Grid g, const float4* __restrict__ pos_r,
-                                                          const float4* __restrict__ vel_r,
+// Between two substeps of one call the state stays in sorted order.  The
+// finalize of substep k (k_finalize_predict) also runs substep k+1's
+// predict: it has the final position and velocity in registers and, with
+// cohesion, the next normals from its own XSPH pass (nrm_r, by slot).  It
+// writes the final position (pos_r: the next substep's start position),
+// the prediction, the cell key and the rank by slot; the predicted velocity
+// is not kept (the velocity pass recomputes it from the positions,
+// shaders/sph_velocity.glsl:12-24).  nrm_canon (when substep k+1 is the
+// call's last) receives the canonical normals the caller's normal buffer
+// holds after that substep.  Bit for bit what a separate finalize and
+// predict would compute: the same float values pass through registers
+// instead of memory.
+__global__ __launch_bounds__(256) void k_finalize_predict(int n, FxParams p, Collider col, Grid g,
+                                                          const float4* __restrict__ pred_s,
+                                                          const float4* __restrict__ vel_s,
+                                                          const float4* __restrict__ dv_s,
                                                           const float4* __restrict__ nrm_r,
-                                                          float4* __restrict__ pred_r, int32_t* __restrict__ key,
-                                                          int32_t* __restrict__ cnt, int32_t* __restrict__ rank,
-                                                          const int32_t* __restrict__ pid,
+                                                          const int32_t* __restrict__ sidx,
+                                                          float4* __restrict__ pos_r, float4* __restrict__ pred_r,
+                                                          int32_t* __restrict__ key, int32_t* __restrict__ cnt,
+                                                          int32_t* __restrict__ rank,
                                                           float4* __restrict__ nrm_canon) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= p.n) return;
+  if (t >= n) return;
+  const float4 pi = pred_s[t], vi = vel_s[t], dv = dv_s[t];
+  float vx = vi.x + dv.x, vy = vi.y + dv.y, vz = vi.z + dv.z;
+  float px = pi.x, py = pi.y, pz = pi.z;
+  finalize_one(p, col, px, py, pz, vx, vy, vz);
+  const float4 pf = make_float4(px, py, pz, 1.0f), vf = make_float4(vx, vy, vz, 0.0f);
+  pos_r[t] = pf;
   const float4 nrm = (p.surface_tension > 0.0f && nrm_r) ? nrm_r[t] : make_float4(0.f, 0.f, 0.f, 0.f);
   float4 vo, pp;
-  predict_one(p, pos_r[t], vel_r[t], nrm, vo, pp);
+  predict_one(p, pf, vf, nrm, vo, pp);
   pred_r[t] = pp;
   int x, y, z;
   cell_coord(g, pp.x, pp.y, pp.z, x, y, z);
   const int k = cell_index(g, x, y, z);
   key[t] = k;
   rank[t] = atomicAdd(&cnt[k], 1);
-  if (nrm_canon) nrm_canon[min(max(pid[t], 0), p.n - 1)] = nrm;
+  if (nrm_canon) nrm_canon[min(max(sidx[t], 0), n - 1)] = nrm;
 }
 
 // Counting-sort scatter of (particle << 32 | previous slot) pairs.
@@ -1233,9 +1247,7 @@ __global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p, Collider 
                                                     const int32_t* __restrict__ sidx,
                                                     float4* __restrict__ delta4, float4* __restrict__ pos4,
                                                     float4* __restrict__ vel4, float4* __restrict__ pos_s,
-                                                    float4* __restrict__ pred4_out,
-                                                    float4* __restrict__ pos_r = nullptr,
-                                                    float4* __restrict__ vel_r = nullptr) {
+                                                    float4* __restrict__ pred4_out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const float4 pi = pred_s[t], vi = vel_s[t], dv = dv_s[t];
@@ -1249,9 +1261,6 @@ __global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p, Collider 
   if (pos4) pos4[i] = pf;
   if (vel4) vel4[i] = vf;
   if (pos_s) pos_s[t] = pf;
-  // the state by slot for the next substep of the same call
-  if (pos_r) pos_r[t] = pf;
-  if (vel_r) vel_r[t] = vf;
 }
 
 // shaders/sph_normal.glsl:24-81 over the previous grid; pred_s.w = the
@@ -1554,30 +1563,25 @@ static int build_grid(const FxParams& p, FxState& s, const Ws& w, bool do_predic
 }
 
 // Grid build of a substep whose state enters by slot of the previous
-// substep of the same fx_step call (see k_predict_resident).  The counting
-// sort carries (particle, previous slot) pairs, so each cell's run is still
-// put in ascending particle order - the same order as build_grid - and the
-// gather reads the previous slots nearly in sequence instead of gathering
-// the canonical arrays.  The reference-format outputs (keys, sorted keys and
-// indices, cell ranges, normals) go to the caller on the call's last
-// substep only.
+// substep of the same fx_step call, predicted already by that substep's
+// k_finalize_predict (prediction in vel_r, keys and ranks by slot, counts in
+// cnt).  The counting sort carries (particle, previous slot) pairs, so each
+// cell's run is still put in ascending particle order - the same order as
+// build_grid - and the gather reads the previous slots nearly in sequence
+// instead of gathering the canonical arrays.  The reference-format outputs
+// (keys, sorted keys and indices, cell ranges) go to the caller on the
+// call's last substep only.
 static int build_grid_resident(const FxParams& p, FxState& s, const Ws& w, int target, bool last_in_call,
                                hipStream_t st) {
   const int n = p.n, nc = p.n_cells;
-  Grid g = make_grid(p);
-  FX_TRY(fill(w.cnt, nc + 1, 0, st));
-  float4* pred_r = w.pred_s[target ^ 1];  // free: the previous substep is finalized
-  const bool cohesion = p.surface_tension > 0.0f;
-  LAUNCH(k_predict_resident, n, st, p, g, (const float4*)w.pos_r, (const float4*)w.vel_r,
-         cohesion ? (const float4*)w.normal_next : (const float4*)nullptr, pred_r, w.key, w.cnt, w.rank,
-         (const int32_t*)w.sidx, (cohesion && last_in_call) ? (float4*)s.normal : (float4*)nullptr);
+  const float4* pred_r = w.vel_r;  // the prediction by slot (see k_finalize_predict)
   FX_TRY(exclusive_scan_i32(w.cnt, w.off, nc, w.partial, st));
   if (last_in_call && s.cell_start && s.cell_end) LAUNCH(k_cell_ranges, nc, st, nc, w.off, s.cell_start, s.cell_end);
   LAUNCH(k_scatter_pairs, n, st, n, (const int32_t*)w.key, (const int32_t*)w.rank, (const int32_t*)w.off,
          (const int32_t*)w.sidx, w.pair64);
   // big-cell queue in the rank array (dead after the scatter)
   FX_TRY(canon_pairs(w.off, w.pair64, nc, w.rank, w.flags, st));
-  LAUNCH(k_gather_pairs, n, st, n, (const uint64_t*)w.pair64, (const int32_t*)w.key, (const float4*)pred_r,
+  LAUNCH(k_gather_pairs, n, st, n, (const uint64_t*)w.pair64, (const int32_t*)w.key, pred_r,
          (const float4*)w.pos_r, w.pred_s[target], w.pos_s, w.dl_s, last_in_call ? s.sorted_key : nullptr, w.skey,
          w.sidx, last_in_call ? s.sorted_idx : nullptr, last_in_call ? s.key : nullptr);
   forget_normals(s.work);
@@ -1601,8 +1605,9 @@ static bool resident_ok(const FxParams& p, const FxState& s) {
 
 // resident_in: the state enters by slot from the previous substep of the
 // call; resident_out: it leaves that way for the next one.
+// next_is_last: with resident_out, the next substep is the call's last.
 static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, bool last_in_call = true,
-                   bool resident_in = false, bool resident_out = false) {
+                   bool resident_in = false, bool resident_out = false, bool next_is_last = false) {
   const int n = p.n;
   Grid g = make_grid(p);
   Consts k = make_consts(p);
@@ -1687,14 +1692,17 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
               w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, (float4*)nullptr,
               L, Scenes{nullptr, 0});
     }
-    if (resident_out)
+    if (resident_out) {
+      // finalize, and the next substep's predict (see k_finalize_predict)
+      FX_TRY(fill(w.cnt, p.n_cells + 1, 0, st));
+      const bool cohesion = p.surface_tension > 0.0f;
+      LAUNCH(k_finalize_predict, n, st, n, p, col, g, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+             (const float4*)w.pred_s[1], cohesion ? (const float4*)w.normal_next : (const float4*)nullptr,
+             (const int32_t*)w.sidx, w.pos_r, w.vel_r, w.key, w.cnt, w.rank,
+             (cohesion && next_is_last) ? (float4*)s.normal : (float4*)nullptr);
+    } else
       LAUNCH(k_finalize_s, n, st, n, p, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
-             (const float4*)w.pred_s[1], w.sidx, (float4*)nullptr, (float4*)nullptr, (float4*)nullptr,
-             (float4*)nullptr, (float4*)nullptr, w.pos_r, w.vel_r);
-    else
-      LAUNCH(k_finalize_s, n, st, n, p, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
-             (const float4*)w.pred_s[1], w.sidx, delta_out, (float4*)s.pos, (float4*)s.vel, pos_s_out, pred_out,
-             (float4*)nullptr, (float4*)nullptr);
+             (const float4*)w.pred_s[1], w.sidx, delta_out, (float4*)s.pos, (float4*)s.vel, pos_s_out, pred_out);
     if (fuse_normals && !resident_out) mark_normals(s.work, s.normal, p, !last_in_call);
   }
   return check_launch("substep");
@@ -2077,7 +2085,8 @@ int fx_step(const FxParams* p, FxState* s, int substeps, void* stream) {
   const bool resident = resident_ok(*p, *s);
   for (int i = 0; i < substeps; ++i) {
     const bool last = i == substeps - 1;
-    FX_TRY(substep(*p, *s, w, (hipStream_t)stream, last, resident && i > 0, resident && !last));
+    FX_TRY(substep(*p, *s, w, (hipStream_t)stream, last, resident && i > 0, resident && !last,
+                   i + 1 == substeps - 1));
   }
   return FX_OK;
 }
