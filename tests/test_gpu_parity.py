@@ -441,10 +441,12 @@ def test_step_equals_single_substeps(fx, scene):
     """fx_step(K) leaves out, in all but its last substep, the outputs the
     next substep overwrites before anything reads them (canonical prediction,
     lambda, correction, the reference-format grid, the sorted final
-    positions) and lets the fused XSPH pass write the next normals in place:
-    every buffer the caller sees after the call is bit-identical to K single
-    fx_substep calls - cohesion off and on, full stencil and neighbour lists -
-    and a second call carries on identically."""
+    positions) and lets the fused XSPH pass write the next normals in place;
+    without neighbour lists the state also stays in sorted order between the
+    substeps: every buffer the caller sees after the call is bit-identical to
+    K single fx_substep calls - cohesion off and on, full stencil and
+    neighbour lists, 4, 3 and 0 Jacobi iterations - and a second call
+    carries on identically."""
     import ctypes
 
     L = fx._lib
@@ -452,22 +454,23 @@ def test_step_equals_single_substeps(fx, scene):
     names = ("pos", "vel", "pred", "lam", "delta", "normal", "key", "sorted_key", "sorted_idx", "cell_start",
              "cell_end")
     saved = L.get_tuning()
+    cases = [(skin, sigma, 4) for skin in (0.0, 0.2) for sigma in (0.0, 0.001)] + [(0.0, 0.001, 3), (0.0, 0.0, 0)]
     try:
-        for skin in (0.0, 0.2):
+        for skin, sigma, iterations in cases:
             L.set_tuning(list_skin=skin, block=saved["block"])
-            for sigma in (0.0, 0.001):
-                p.surface_tension = sigma
-                a = GpuState(fx, p, collider=col, pos=st.pos, vel=st.vel)
-                b = GpuState(fx, p, collider=col, pos=st.pos, vel=st.vel)
-                a.run("load_state")
-                b.run("load_state")
-                for call in range(2):
-                    L.check(L.lib().fx_step(ctypes.byref(p), ctypes.byref(a.state), 3, a.stream()), "fx_step")
-                    a.torch.cuda.synchronize()
-                    for _ in range(3):
-                        b.run("substep")
-                    for name in names:
-                        assert np.array_equal(a.get(name), b.get(name)), (skin, sigma, call, name)
+            p.surface_tension = sigma
+            p.iterations = iterations
+            a = GpuState(fx, p, collider=col, pos=st.pos, vel=st.vel)
+            b = GpuState(fx, p, collider=col, pos=st.pos, vel=st.vel)
+            a.run("load_state")
+            b.run("load_state")
+            for call in range(2):
+                L.check(L.lib().fx_step(ctypes.byref(p), ctypes.byref(a.state), 3, a.stream()), "fx_step")
+                a.torch.cuda.synchronize()
+                for _ in range(3):
+                    b.run("substep")
+                for name in names:
+                    assert np.array_equal(a.get(name), b.get(name)), (skin, sigma, iterations, call, name)
     finally:
         L.set_tuning(list_skin=saved["list_skin"], block=saved["block"])
 
