@@ -186,7 +186,11 @@ int fx_build_grid(const FxParams* p, FxState* s, void* stream);
  * fx_substep runs one substep in the reference's pass order (normal if
  * surface_tension > 0, predict, grid build, iterations x (lambda, delta,
  * apply), velocity, xsph, finalize) with passes fused where the data flow
- * allows; fx_step runs `substeps` of them (solver/sph.py:529-530). */
+ * allows; fx_step runs `substeps` of them (solver/sph.py:529-530).  After
+ * fx_step every caller buffer holds what `substeps` fx_substep calls would
+ * leave, bit for bit; in between, the substeps keep their state in the
+ * workspace in sorted order and only the last writes the canonical buffers
+ * and the optional scratch copies. */
 int fx_substep(const FxParams* p, FxState* s, void* stream);
 int fx_step(const FxParams* p, FxState* s, int substeps, void* stream);
 
