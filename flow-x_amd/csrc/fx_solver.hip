@@ -719,10 +719,11 @@ constexpr int VISIT_STREAM = 4;        // one stream per lane; batches run on ac
 constexpr int VISIT_MASKED_LOADS = 8;  // with VISIT_STREAM: empty batch slots load nothing
 constexpr int VISIT_ROW_BATCH = 16;    // with VISIT_STREAM: batches stay in a row
 // Kernel template bit: walk row by row.  A scene under kStreamMinParticles
-// (about 8 waves per CU) is latency-bound - few waves hide nothing - and the
-// stream walks' bookkeeping lengthens every wave's chain: at 25,600
-// particles lambda / delta / XSPH take 51 / 48 / 58 us row by row against
-// 58 / 59 / 74 us streamed.
+// (about 8 waves per CU) leaves most SIMDs with one wave or none, so a pass
+// lasts as long as one wave's instruction stream (~330 cycles per candidate
+// at 25,600 particles), and the stream walks' bookkeeping lengthens it: at
+// 25,600 particles lambda / delta / XSPH take 51 / 48 / 58 us row by row
+// against 58 / 59 / 74 us streamed.
 constexpr int kPlainWalk = 64;
 constexpr int kStreamMinParticles = 1 << 17;
 

@@ -89,3 +89,21 @@ def test_autotune_file_is_applied(fx, tmp_path):
         assert autotune.apply(tmp_path / "missing.json") is None
     finally:
         _lib.set_tuning(list_skin=saved["list_skin"], block=saved["block"])
+
+
+def test_tuning_round_trip_and_rejects_bad_values(fx):
+    """fx_set_tuning stores every FxTuning field and refuses out-of-range
+    values without changing the stored tuning."""
+    from flowx_amd import _lib
+
+    saved = _lib.get_tuning()
+    try:
+        _lib.set_tuning(list_skin=0.1, block=256)
+        t = _lib.get_tuning()
+        assert t["block"] == 256 and abs(t["list_skin"] - 0.1) < 1e-7
+        for bad in ({"block": 96}, {"list_skin": -0.5}, {"list_skin": 2.0}):
+            with pytest.raises(RuntimeError):
+                _lib.set_tuning(**bad)
+            assert _lib.get_tuning() == t
+    finally:
+        _lib.set_tuning(**saved)
