@@ -968,6 +968,167 @@ __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restr
   }
 }
 
+// Small scenes, full stencil, four lanes per particle (kernel template bit
+// kQuadWalk).  Under kStreamMinParticles most SIMDs hold one wave or none,
+// and a pass lasts as long as one wave's instruction stream; four lanes per
+// particle quarter each stream and fill four times the SIMDs.  The quad walks
+// its particle's rows exactly as visit's row-by-row path does (same runs, same
+// per-cell cut past max_scan); in each round lane j takes slot k + j of the
+// run, computes its term (a zero term past the run's end or out of range),
+// and the quad folds the four terms in slot order through DPP broadcasts, each
+// lane adding lane 0's, then lane 1's, ... term to its own copy of the sums.
+// The terms and the fold are the same explicitly rounded operations the
+// one-lane walks use, and folding a zero term (a zero summand, or a zero
+// factor of a fused multiply-add) equals skipping the candidate: the sums
+// start at +0 and round to nearest, so they are never -0, and x + (+-0) = x.
+// So the four copies and the one-lane walks agree bit for bit.  tm(q, data,
+// valid) returns the
+// term of slot q; fold(term) adds one term.  All four lanes of a quad run the
+// same control flow (same cell, same runs), which the DPP reads need.
+constexpr int kQuadWalk = 128;
+constexpr int kQuadBatch = 4;  // rounds of loads in flight per lane
+
+template <int J>
+__device__ __forceinline__ float quad_lane(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), J * 0x55, 0xF, 0xF,
+                                                              false));
+}
+
+template <class LD, class TM, class FD>
+__device__ __forceinline__ void visit_quad(const Grid& g, const int32_t* __restrict__ off, int max_scan, int bx,
+                                           int by, int bz, int j, LD&& ld, TM&& tm, FD&& fold) {
+  constexpr int B = kQuadBatch;
+  const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
+  auto run = [&](int s, int e) {
+#pragma unroll 1
+    for (int k = s; k < e; k += 4 * B) {
+      decltype(ld(0)) d[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) d[u] = ld(min(k + 4 * u + j, e - 1));
+      decltype(tm(0, d[0], false)) tt[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) tt[u] = tm(k + 4 * u + j, d[u], k + 4 * u + j < e);
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        if (k + 4 * u < e) {  // the same for the four lanes
+          fold(tt[u].template lane<0>());
+          fold(tt[u].template lane<1>());
+          fold(tt[u].template lane<2>());
+          fold(tt[u].template lane<3>());
+        }
+      }
+    }
+  };
+  auto row = [&](int r, int& c0, int& s, int& e) {
+    const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
+    if (z < 0 || z >= g.cz || y < 0 || y >= g.cy) {
+      c0 = -1;
+      s = e = 0;
+      return;
+    }
+    c0 = (z * g.cy + y) * g.cx + x0;
+    s = off[c0];
+    e = off[c0 + (x1 - x0) + 1];
+  };
+  int c0n, sn, en;
+  row(0, c0n, sn, en);
+#pragma unroll 1
+  for (int r = 0; r < 9; ++r) {
+    const int c0 = c0n, s = sn, e = en;
+    if (r < 8) row(r + 1, c0n, sn, en);
+    if (c0 < 0) continue;
+    if (e - s <= max_scan) {
+      run(s, e);
+    } else {
+      for (int c = c0; c <= c0 + (x1 - x0); ++c) {
+        const int cs = off[c], ce = min(off[c + 1], cs + max_scan);
+        run(cs, ce);
+      }
+    }
+  }
+}
+
+// Lambda's per-candidate terms and their fold, shared by every walk: the
+// density term m ((coef d) d) d (the reference's operations, so the density
+// sum stays bit-exact), and the gradient c d, c = spiky (h - r)^2 / r without
+// the mass (hoisted), with its squared norm.  A term that does not contribute
+// has dens = 0 and c = 0 (its gradient and norm are then 0 too).
+__device__ __forceinline__ float lam_dens(const Consts& k, float r2) {
+#pragma clang fp contract(off)
+  const float d = k.h2 - r2;
+  return k.mass * (((k.poly6 * d) * d) * d);
+}
+__device__ __forceinline__ float lam_coef(const Consts& k, float r2) {
+#pragma clang fp contract(off)
+  const float inv_r = __builtin_amdgcn_rsqf(r2);
+  const float w = __builtin_fmaf(-r2, inv_r, k.h);  // h - r
+  return ((k.spiky * w) * w) * inv_r;
+}
+struct LamTerm {
+  float dens, ax, ay, az, norm;
+  template <int J>
+  __device__ __forceinline__ LamTerm lane() const {
+    return {quad_lane<J>(dens), quad_lane<J>(ax), quad_lane<J>(ay), quad_lane<J>(az), quad_lane<J>(norm)};
+  }
+};
+// the gradient c d and its squared norm, given c (0 for no term)
+__device__ __forceinline__ LamTerm lam_grad(float dens, float c, float dx, float dy, float dz) {
+#pragma clang fp contract(off)
+  const float ax = c * dx, ay = c * dy, az = c * dz;
+  return {dens, ax, ay, az, __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))};
+}
+struct LamSums {
+  float dens = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f, gn2 = 0.0f;
+  __device__ __forceinline__ void add_dens(float t) {
+#pragma clang fp contract(off)
+    dens = dens + t;
+  }
+  __device__ __forceinline__ void add_grad(const LamTerm& t) {
+#pragma clang fp contract(off)
+    gx = gx + t.ax;
+    gy = gy + t.ay;
+    gz = gz + t.az;
+    gn2 = gn2 + t.norm;
+  }
+  __device__ __forceinline__ void add(const LamTerm& t) {
+    add_dens(t.dens);
+    add_grad(t);
+  }
+};
+
+// Delta's per-candidate correction c d, c = (lambda_i + lambda_j + s_corr)
+// spiky (h - r)^2 / r, and its fold (one fused multiply-add per component),
+// shared by every walk.  A term that does not contribute has c = 0.
+struct DelTerm {
+  float c, dx, dy, dz;
+  template <int J>
+  __device__ __forceinline__ DelTerm lane() const {
+    return {quad_lane<J>(c), quad_lane<J>(dx), quad_lane<J>(dy), quad_lane<J>(dz)};
+  }
+};
+__device__ __forceinline__ float del_coef(const Consts& k, float li, float lj, float inv_wq, bool scorr, float r2) {
+#pragma clang fp contract(off)
+  const float inv_r = __builtin_amdgcn_rsqf(r2);
+  float sc = 0.0f;
+  if (scorr) {
+    const float d = k.h2 - r2;
+    const float ratio = (((k.poly6 * d) * d) * d) * inv_wq;
+    const float r4 = ratio * ratio;
+    sc = -k.scorr_k * (r4 * r4);
+  }
+  const float l = (li + lj) + sc;
+  const float w = __builtin_fmaf(-r2, inv_r, k.h);  // h - r
+  return (l * ((k.spiky * w) * w)) * inv_r;
+}
+struct DelSums {
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+  __device__ __forceinline__ void add(const DelTerm& t) {
+    ox = __builtin_fmaf(t.dx, t.c, ox);
+    oy = __builtin_fmaf(t.dy, t.c, oy);
+    oz = __builtin_fmaf(t.dz, t.c, oz);
+  }
+};
+
 // shaders/sph_lambda.glsl:17-77.  Writes dl_s[t] = (max(rho,1e-6), lambda),
 // pred_s[t].w = lambda (read by the delta pass with the position), and the
 // canonical lambda4 row if requested.  The density sum is bit exact
@@ -981,7 +1142,9 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
                                                 const int32_t* __restrict__ sidx,
                                                 float4* __restrict__ lambda4, Lists L, Scenes S) {
   constexpr int MODE = V & 3;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr bool QUAD = (V & kQuadWalk) != 0;
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = QUAD ? gt >> 2 : gt;
   if (t >= n) return;
   Grid g = g_;
   Consts k = k_;
@@ -995,31 +1158,32 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
   const float4 pi = pred_s[t];
   int bx, by, bz;
   cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
-  float dens = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f, gn2 = 0.0f;
+  LamSums sum;
   {
-#pragma clang fp contract(fast)
-    const float h = k.h, h2 = k.h2, r2l = L.r2_list;
+    const float h2 = k.h2, r2l = L.r2_list;
     auto ld = [&](int q) { return pred_s[q]; };
-    visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM | VISIT_ROW_BATCH)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
-      const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-      const float r2 = dist2_exact(dx, dy, dz);
-      if (r2 < h2) {
-        dens = poly6_accum_exact(dens, k.mass, k.poly6, h2 - r2);
-        if (q != t && r2 > 1e-12f) {
-          const float inv_r = __builtin_amdgcn_rsqf(r2);
-          const float r = r2 * inv_r;
-          const float w = h - r;
-          const float c = k.spiky * w * w * inv_r;
-          const float ax = c * dx, ay = c * dy, az = c * dz;
-          gx += ax;
-          gy += ay;
-          gz += az;
-          gn2 += ax * ax + ay * ay + az * az;
+    if constexpr (QUAD) {
+      auto tm = [&](int q, const float4& pj, bool valid) {
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2_exact(dx, dy, dz);
+        const bool in = valid && r2 < h2, grad = in && q != t && r2 > 1e-12f;
+        return lam_grad(in ? lam_dens(k, r2) : 0.0f, grad ? lam_coef(k, r2) : 0.0f, dx, dy, dz);
+      };
+      visit_quad(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const LamTerm& tt) { sum.add(tt); });
+      if (gt & 3) return;
+    } else {
+      visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM | VISIT_ROW_BATCH)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2_exact(dx, dy, dz);
+        if (r2 < h2) {
+          sum.add_dens(lam_dens(k, r2));
+          if (q != t && r2 > 1e-12f) sum.add_grad(lam_grad(0.0f, lam_coef(k, r2), dx, dy, dz));
         }
-      }
-      return r2 < r2l;
-    });
+        return r2 < r2l;
+      });
+    }
   }
+  float dens = sum.dens, gx = sum.gx, gy = sum.gy, gz = sum.gz, gn2 = sum.gn2;
   // the mass factor of the gradient terms is hoisted out of the sums
   gx *= k.mass;
   gy *= k.mass;
@@ -1051,7 +1215,9 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
                                                float4* __restrict__ delta4, Lists L,
                                                const int32_t* __restrict__ skey, Scenes S) {
   constexpr int MODE = V & 3;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr bool QUAD = (V & kQuadWalk) != 0;
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = QUAD ? gt >> 2 : gt;
   if (t < n) {
     Grid g = g_;
     Consts k = k_;
@@ -1066,35 +1232,31 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
     const float li = pi.w;
     int bx, by, bz;
     cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
-    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    DelSums sum;
     {
-#pragma clang fp contract(fast)
-      const float h = k.h, h2 = k.h2;
+      const float h2 = k.h2;
       const bool scorr = k.scorr_k > 0.0f && k.w_q > 1e-9f;
       const float inv_wq = scorr ? 1.0f / k.w_q : 0.0f;
       auto ld = [&](int q) { return pred_in[q]; };
-      visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dist2_exact(dx, dy, dz);
-        if (r2 < h2 && r2 > 1e-12f && q != t) {
-          const float inv_r = __builtin_amdgcn_rsqf(r2);
-          const float r = r2 * inv_r;
-          float sc = 0.0f;
-          if (scorr) {
-            const float d = h2 - r2;
-            const float ratio = k.poly6 * d * d * d * inv_wq;
-            const float r4 = ratio * ratio;
-            sc = -k.scorr_k * (r4 * r4);
-          }
-          const float w = h - r;
-          const float c = (li + pj.w + sc) * (k.spiky * w * w) * inv_r;
-          ox += c * dx;
-          oy += c * dy;
-          oz += c * dz;
-        }
-        return false;
-      });
+      if constexpr (QUAD) {
+        auto tm = [&](int q, const float4& pj, bool valid) {
+          const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+          const float r2 = dist2_exact(dx, dy, dz);
+          const bool in = valid && r2 < h2 && r2 > 1e-12f && q != t;
+          return DelTerm{in ? del_coef(k, li, pj.w, inv_wq, scorr, r2) : 0.0f, dx, dy, dz};
+        };
+        visit_quad(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const DelTerm& tt) { sum.add(tt); });
+        if (gt & 3) return;
+      } else {
+        visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+          const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+          const float r2 = dist2_exact(dx, dy, dz);
+          if (r2 < h2 && r2 > 1e-12f && q != t) sum.add(DelTerm{del_coef(k, li, pj.w, inv_wq, scorr, r2), dx, dy, dz});
+          return false;
+        });
+      }
     }
+    const float ox = sum.ox, oy = sum.oy, oz = sum.oz;
     const float ddx = (ox * k.delta_scale) / k.rho0, ddy = (oy * k.delta_scale) / k.rho0,
                 ddz = (oz * k.delta_scale) / k.rho0;
     const float2 dl = dl_s[t];
@@ -1463,6 +1625,15 @@ static int validate(const FxParams* p, const FxState* s, bool need_work) {
   FX_LAUNCH(kernel, dim3(grid_for(n, g_tuning.block)), dim3(g_tuning.block), st, __VA_ARGS__)
 // a full-stencil neighbour pass with the walk chosen by scene size
 // (V unparenthesised: the launch's profile label is the kernel text)
+// lambda and delta: four lanes per particle under kStreamMinParticles
+#define QLAUNCH(kernel, V, n, st, ...)                                                                    \
+  do {                                                                                                    \
+    if ((n) >= kStreamMinParticles)                                                                       \
+      NLAUNCH(kernel<V>, n, st, __VA_ARGS__);                                                             \
+    else                                                                                                  \
+      FX_LAUNCH(kernel<V + kQuadWalk>, dim3(grid_for(4LL * (n), g_tuning.block)), dim3(g_tuning.block), st, \
+                __VA_ARGS__);                                                                             \
+  } while (0)
 #define WLAUNCH(kernel, V, n, st, ...)                     \
   do {                                                     \
     if ((n) >= kStreamMinParticles)                        \
@@ -1654,7 +1825,7 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
     const bool last = it == iters - 1;
     float4* lam_out = (last && last_in_call && s.lambda) ? (float4*)s.lambda : (float4*)nullptr;
     if (!lists)
-      WLAUNCH(k_lambda, VISIT_FULL, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
+      QLAUNCH(k_lambda, VISIT_FULL, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else if (it == 0)
       NLAUNCH(k_lambda<VISIT_BUILD>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else
@@ -1664,7 +1835,7 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
               w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
               (float4*)nullptr, L, (const int32_t*)w.skey, Scenes{nullptr, 0});
     else
-      WLAUNCH(k_delta, VISIT_FULL, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
+      QLAUNCH(k_delta, VISIT_FULL, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
               w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
               (float4*)nullptr, L, (const int32_t*)nullptr, Scenes{nullptr, 0});
     cur ^= 1;
@@ -1796,7 +1967,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
         LAUNCH(k_gather, n, st, n, w.sidx, (const int32_t*)nullptr, (const float4*)s.pred, (const float4*)nullptr,
                (const float4*)nullptr, w.pred_s[0], (float4*)nullptr, (float2*)nullptr,
                (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
-        WLAUNCH(k_lambda, VISIT_FULL, n, st, n, g, k, w.pred_s[0], w.off, w.dl_s, w.sidx, (float4*)s.lambda, L0, Scenes{nullptr, 0});
+        QLAUNCH(k_lambda, VISIT_FULL, n, st, n, g, k, w.pred_s[0], w.off, w.dl_s, w.sidx, (float4*)s.lambda, L0, Scenes{nullptr, 0});
       }
       break;
     case FX_PASS_DELTA:
@@ -1810,7 +1981,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
                (const float4*)s.lambda, w.pred_s[0], (float4*)nullptr, w.dl_s, (int32_t*)nullptr,
                (int32_t*)nullptr, (int32_t*)nullptr);
         LAUNCH(k_w_from_lambda, n, st, n, w.pred_s[0], (const float2*)w.dl_s);
-        WLAUNCH(k_delta, VISIT_FULL, n, st, n, g, k, (const float4*)w.pred_s[0], (const float2*)w.dl_s, w.off,
+        QLAUNCH(k_delta, VISIT_FULL, n, st, n, g, k, (const float4*)w.pred_s[0], (const float2*)w.dl_s, w.off,
                 (float4*)nullptr, (const float4*)nullptr, (float4*)nullptr, w.sidx, (float4*)s.delta, L0,
                 (const int32_t*)nullptr, Scenes{nullptr, 0});
       }
