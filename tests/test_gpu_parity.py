@@ -192,6 +192,33 @@ def test_finalize_bit_exact(fx, oracle, scene):
     assert np.array_equal(g.get("vel"), vel_ref)
 
 
+def test_finalize_push_out_at_the_grid_edge_bit_exact(fx, oracle, scene):
+    """A collider in the domain's corner, on the floor: the 5^3 search around
+    a particle inside it reaches past the collider grid on three sides, where
+    voxels are skipped (collision/__init__.py grid, shaders/sph_finalize.glsl
+    search), and the nearest free voxel is found among the rest."""
+    from flowx_amd.collision import grid_geometry, occupancy_cpu
+
+    cfg, p, st = scene["cfg"], scene["p"], scene["st"]
+    origin, voxel, dims = grid_geometry(cfg.lo, cfg.hi, cfg.extra["collider_resolution"])
+    col = occupancy_cpu(origin, voxel, dims, [("box", (-1.0, -1.0, 0.0), (-0.62, -0.58, 0.35))]).reshape(-1)
+    assert col.sum() > 0
+    pred = st.pred.copy()
+    rng = np.random.default_rng(2)
+    k = rng.choice(len(pred), 2000, replace=False)
+    pred[k, 0] = rng.uniform(-0.999, -0.63, 2000)
+    pred[k, 1] = rng.uniform(-0.999, -0.59, 2000)
+    pred[k, 2] = rng.uniform(0.001, 0.34, 2000)
+    delta = (rng.standard_normal((len(pred), 4)) * 0.01).astype(np.float32)
+    delta[:, 3] = 0
+    pos_ref, vel_ref = oracle.finalize(p, st.vel, delta, pred, col)
+    assert np.abs(pos_ref[k, :3] - pred[k, :3]).max() > 0.01  # particles were pushed out
+    g = GpuState(fx, p, collider=col, pos=st.pos, vel=st.vel, pred=pred, delta=delta)
+    g.run("finalize")
+    assert np.array_equal(g.get("pos"), pos_ref)
+    assert np.array_equal(g.get("vel"), vel_ref)
+
+
 def test_normal(fx, oracle, scene):
     p, st, n = scene["p"], scene["st"], scene["n"]
     keys, idx, start, end = _stable_grid(oracle, p, st.pred, n)
