@@ -1140,6 +1140,56 @@ struct DelSums {
   }
 };
 
+// XSPH's per-candidate terms (the velocity difference weighted by
+// ((d d) d) / rho_j, and with the normals the gradient weight (d d) / rho_j
+// and the Laplacian weight (h - r) / rho_j) and their fold, one fused
+// multiply-add per sum, shared by every walk.  A term that does not
+// contribute is all zeros.
+struct XsTerm {
+  float w, dvx, dvy, dvz, gsc, dx, dy, dz, hr, irho;
+  template <int J>
+  __device__ __forceinline__ XsTerm lane() const {
+    return {quad_lane<J>(w),   quad_lane<J>(dvx), quad_lane<J>(dvy), quad_lane<J>(dvz), quad_lane<J>(gsc),
+            quad_lane<J>(dx),  quad_lane<J>(dy),  quad_lane<J>(dz),  quad_lane<J>(hr),  quad_lane<J>(irho)};
+  }
+};
+template <bool NORMAL>
+__device__ __forceinline__ XsTerm xs_term(const Consts& k, const float4& vi, const float4& vj, float dx, float dy,
+                                          float dz, float r2) {
+#pragma clang fp contract(off)
+  XsTerm o;
+  const float d = k.h2 - r2;
+  o.irho = __builtin_amdgcn_rcpf(fmaxf(vj.w, 1e-6f));
+  o.w = ((d * d) * d) * o.irho;
+  o.dvx = vj.x - vi.x;
+  o.dvy = vj.y - vi.y;
+  o.dvz = vj.z - vi.z;
+  o.gsc = o.dx = o.dy = o.dz = o.hr = 0.0f;
+  if (NORMAL) {  // k_normal's pair term (pj.w is the same density)
+    o.gsc = (d * d) * o.irho;
+    o.dx = dx;
+    o.dy = dy;
+    o.dz = dz;
+    o.hr = __builtin_fmaf(-r2, __builtin_amdgcn_rsqf(r2), k.h);  // h - r
+  }
+  return o;
+}
+template <bool NORMAL>
+struct XsSums {
+  float ax = 0.0f, ay = 0.0f, az = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f, lap = 0.0f;
+  __device__ __forceinline__ void add(const XsTerm& t) {
+    ax = __builtin_fmaf(t.dvx, t.w, ax);
+    ay = __builtin_fmaf(t.dvy, t.w, ay);
+    az = __builtin_fmaf(t.dvz, t.w, az);
+    if (NORMAL) {
+      nx = __builtin_fmaf(t.dx, t.gsc, nx);
+      ny = __builtin_fmaf(t.dy, t.gsc, ny);
+      nz = __builtin_fmaf(t.dz, t.gsc, nz);
+      lap = __builtin_fmaf(t.hr, t.irho, lap);
+    }
+  }
+};
+
 // shaders/sph_lambda.glsl:17-77.  Writes dl_s[t] = (max(rho,1e-6), lambda),
 // pred_s[t].w = lambda (read by the delta pass with the position), and the
 // canonical lambda4 row if requested.  The density sum is bit exact
@@ -1319,7 +1369,9 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
   constexpr bool SPLIT = (V & 16) != 0;  // dv in sorted order for k_finalize_s
   constexpr bool NORMAL = (V & 32) != 0;  // also the next substep's normal pass (k_normal)
   constexpr int MODE = V & 3;
-  const int t = xcd_block() * blockDim.x + threadIdx.x;
+  constexpr bool QUAD = (V & kQuadWalk) != 0;
+  const int gt = xcd_block() * blockDim.x + threadIdx.x;
+  const int t = QUAD ? gt >> 2 : gt;
   if (t >= n) return;
   FxParams p = p_;
   Grid g = g_;
@@ -1338,45 +1390,61 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
   const float4 vi = vel_s[t];
   int bx, by, bz;
   cell_coord(g, pi.x, pi.y, pi.z, bx, by, bz);
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  float nx = 0.0f, ny = 0.0f, nz = 0.0f, lap = 0.0f;
+  XsSums<NORMAL> sum;
   {
-#pragma clang fp contract(fast)
     const float h2 = k.h2;
-    // stage 1 loads positions; stage 2 loads velocities only for candidates
-    // within h (predicated, still before any use); stage 3 accumulates
-    auto ld = [&](int q) {
-      PV c;
-      c.p = pred_s[q];
-      return c;
-    };
-    auto rf = [&](int q, PV& c) {
-      c.r2 = dist2_exact(pi.x - c.p.x, pi.y - c.p.y, pi.z - c.p.z);
-      if (c.r2 < h2 && q != t) c.v = vel_s[q];
-    };
-    visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM | VISIT_MASKED_LOADS)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
-      const float4 pj = c.p, vj = c.v;
-      const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-      const float r2 = c.r2;
-      if (r2 < h2 && q != t) {
-        const float d = h2 - r2;
-        const float inv_rho = __builtin_amdgcn_rcpf(fmaxf(vj.w, 1e-6f));
-        const float w = d * d * d * inv_rho;
-        ax += (vj.x - vi.x) * w;
-        ay += (vj.y - vi.y) * w;
-        az += (vj.z - vi.z) * w;
-        if (NORMAL) {  // k_normal's pair term (pj.w is the same density)
-          const float gsc = d * d * inv_rho;
-          nx += gsc * dx;
-          ny += gsc * dy;
-          nz += gsc * dz;
-          const float r = r2 * __builtin_amdgcn_rsqf(r2);
-          lap += (k.h - r) * inv_rho;
+    if constexpr (QUAD) {
+      // each lane loads its candidate's velocity too (valid slot, finite),
+      // so a term that does not contribute is a clean zero
+      auto ld = [&](int q) { return q; };
+      auto tm = [&](int q, int qc, bool valid) {
+        const float4 pj = pred_s[qc], vj = vel_s[qc];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2_exact(dx, dy, dz);
+        const XsTerm z{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        return valid && r2 < h2 && q != t ? xs_term<NORMAL>(k, vi, vj, dx, dy, dz, r2) : z;
+      };
+      visit_quad(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const XsTerm& tt) { sum.add(tt); });
+      if (gt & 3) return;
+    } else {
+      // stage 1 loads positions; stage 2 loads velocities only for candidates
+      // within h (predicated, still before any use); stage 3 accumulates
+      auto ld = [&](int q) {
+        PV c;
+        c.p = pred_s[q];
+        return c;
+      };
+      auto rf = [&](int q, PV& c) {
+        c.r2 = dist2_exact(pi.x - c.p.x, pi.y - c.p.y, pi.z - c.p.z);
+        if (c.r2 < h2 && q != t) c.v = vel_s[q];
+      };
+      // xs_term and XsSums::add, written out so each sum is updated as soon
+      // as its weight is known (fewer live registers; the same operations)
+      visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM | VISIT_MASKED_LOADS)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
+#pragma clang fp contract(off)
+        const float4 vj = c.v;
+        const float dx = pi.x - c.p.x, dy = pi.y - c.p.y, dz = pi.z - c.p.z;
+        const float r2 = c.r2;
+        if (r2 < h2 && q != t) {
+          const float d = h2 - r2;
+          const float irho = __builtin_amdgcn_rcpf(fmaxf(vj.w, 1e-6f));
+          const float w = ((d * d) * d) * irho;
+          sum.ax = __builtin_fmaf(vj.x - vi.x, w, sum.ax);
+          sum.ay = __builtin_fmaf(vj.y - vi.y, w, sum.ay);
+          sum.az = __builtin_fmaf(vj.z - vi.z, w, sum.az);
+          if (NORMAL) {
+            const float gsc = (d * d) * irho;
+            sum.nx = __builtin_fmaf(dx, gsc, sum.nx);
+            sum.ny = __builtin_fmaf(dy, gsc, sum.ny);
+            sum.nz = __builtin_fmaf(dz, gsc, sum.nz);
+            sum.lap = __builtin_fmaf(__builtin_fmaf(-r2, __builtin_amdgcn_rsqf(r2), k.h), irho, sum.lap);
+          }
         }
-      }
-      return false;
-    }, rf);
+        return false;
+      }, rf);
+    }
   }
+  float ax = sum.ax, ay = sum.ay, az = sum.az, nx = sum.nx, ny = sum.ny, nz = sum.nz, lap = sum.lap;
   if (NORMAL) {
     const float gfac = k.mass * (k.poly6 * -6.0f);
     nx *= gfac;
@@ -1867,11 +1935,11 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
       // resident: by slot for the next substep; last of the call: for the
       // next call; otherwise straight into the caller's buffer
       float4* nrm = (resident_out || last_in_call) ? w.normal_next : (float4*)s.normal;
-      WLAUNCH(k_xsph, 32 + 16 + VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
+      QLAUNCH(k_xsph, 32 + 16 + VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
               (const float4*)w.vel_s, w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr,
               (float4*)nullptr, (float4*)nullptr, L, Scenes{nullptr, 0}, nrm, resident_out ? 1 : 0);
     } else {
-      WLAUNCH(k_xsph, 16 + VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+      QLAUNCH(k_xsph, 16 + VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
               w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, (float4*)nullptr,
               L, Scenes{nullptr, 0});
     }
@@ -2017,7 +2085,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
       if (n > 0) {
         LAUNCH(k_gather_xsph, n, st, n, w.sidx, (const float4*)s.pred, (const float4*)s.vel,
                (const float4*)s.lambda, w.pred_s[0], w.vel_s);
-        WLAUNCH(k_xsph, VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
+        QLAUNCH(k_xsph, VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
                 (const float4*)w.vel_s, w.off, w.sidx, (float4*)s.delta, (float4*)nullptr,
                 (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, L0, Scenes{nullptr, 0});
       }
