@@ -729,13 +729,11 @@ enum { VISIT_FULL = 0, VISIT_BUILD = 1, VISIT_LIST = 2 };
 constexpr int VISIT_STREAM = 4;        // one stream per lane; batches run on across row ends
 constexpr int VISIT_MASKED_LOADS = 8;  // with VISIT_STREAM: empty batch slots load nothing
 constexpr int VISIT_ROW_BATCH = 16;    // with VISIT_STREAM: batches stay in a row
-// Kernel template bit: walk row by row.  A scene under kStreamMinParticles
-// (about 8 waves per CU) leaves most SIMDs with one wave or none, so a pass
-// lasts as long as one wave's instruction stream (~330 cycles per candidate
-// at 25,600 particles), and the stream walks' bookkeeping lengthens it: at
-// 25,600 particles lambda / delta / XSPH take 51 / 48 / 58 us row by row
-// against 58 / 59 / 74 us streamed.
-constexpr int kPlainWalk = 64;
+// A scene under kStreamMinParticles (about 8 waves per CU at one lane per
+// particle) leaves most SIMDs with one wave or none, so a pass lasts as long
+// as one wave's instruction stream (~330 cycles per candidate at 25,600
+// particles); such scenes run the quad walk (visit_quad: four lanes per
+// particle), larger ones the stream walks.
 constexpr int kStreamMinParticles = 1 << 17;
 
 struct PV {
@@ -861,11 +859,10 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
     }
   }
   if (M == VISIT_FULL) {
-    // row by row: small scenes (latency-bound: deeper batches shorten each
-    // lane's chain of loads; dam break at 25,600 particles with 4 / 8 / 12
-    // loads in flight: 57.0 / 55.5 / 57.4 ms), and the stream walks' lanes
-    // with a row past max_scan, which keep FX_BATCH - this path's registers
-    // count against the whole kernel (8 would take XSPH from 73 to 107 VGPRs)
+    // row by row: passes without a stream walk (the normal pass), and the
+    // stream walks' lanes with a row past max_scan, which keep FX_BATCH -
+    // this path's registers count against the whole kernel (8 would take
+    // XSPH from 73 to 107 VGPRs)
     constexpr int kPlainBatch = (MODE & VISIT_STREAM) ? FX_BATCH : 8;
     // the next row's run bounds are loaded while the current row is scanned
     auto row = [&](int r, int& c0, int& s, int& e) {
@@ -1233,7 +1230,7 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
       visit_quad(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const LamTerm& tt) { sum.add(tt); });
       if (gt & 3) return;
     } else {
-      visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM | VISIT_ROW_BATCH)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+      visit_auto<MODE | VISIT_STREAM | VISIT_ROW_BATCH>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2_exact(dx, dy, dz);
         if (r2 < h2) {
@@ -1309,7 +1306,7 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
         visit_quad(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const DelTerm& tt) { sum.add(tt); });
         if (gt & 3) return;
       } else {
-        visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+        visit_auto<MODE | VISIT_STREAM>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
           const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
           const float r2 = dist2_exact(dx, dy, dz);
           if (r2 < h2 && r2 > 1e-12f && q != t) sum.add(DelTerm{del_coef(k, li, pj.w, inv_wq, scorr, r2), dx, dy, dz});
@@ -1420,7 +1417,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
       };
       // xs_term and XsSums::add, written out so each sum is updated as soon
       // as its weight is known (fewer live registers; the same operations)
-      visit_auto<MODE | ((V & kPlainWalk) ? 0 : VISIT_STREAM | VISIT_MASKED_LOADS)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
+      visit_auto<MODE | VISIT_STREAM | VISIT_MASKED_LOADS>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
 #pragma clang fp contract(off)
         const float4 vj = c.v;
         const float dx = pi.x - c.p.x, dy = pi.y - c.p.y, dz = pi.z - c.p.z;
@@ -1702,9 +1699,9 @@ static int validate(const FxParams* p, const FxState* s, bool need_work) {
 // neighbour passes use the tuned block size (64, 128 or 256)
 #define NLAUNCH(kernel, n, st, ...) \
   FX_LAUNCH(kernel, dim3(grid_for(n, g_tuning.block)), dim3(g_tuning.block), st, __VA_ARGS__)
-// a full-stencil neighbour pass with the walk chosen by scene size
-// (V unparenthesised: the launch's profile label is the kernel text)
-// lambda and delta: four lanes per particle under kStreamMinParticles
+// a full-stencil neighbour pass with the walk chosen by scene size: four
+// lanes per particle under kStreamMinParticles (V unparenthesised: the
+// launch's profile label is the kernel text)
 #define QLAUNCH(kernel, V, n, st, ...)                                                                    \
   do {                                                                                                    \
     if ((n) >= kStreamMinParticles)                                                                       \
@@ -1712,13 +1709,6 @@ static int validate(const FxParams* p, const FxState* s, bool need_work) {
     else                                                                                                  \
       FX_LAUNCH(kernel<V + kQuadWalk>, dim3(grid_for(4LL * (n), g_tuning.block)), dim3(g_tuning.block), st, \
                 __VA_ARGS__);                                                                             \
-  } while (0)
-#define WLAUNCH(kernel, V, n, st, ...)                     \
-  do {                                                     \
-    if ((n) >= kStreamMinParticles)                        \
-      NLAUNCH(kernel<V>, n, st, __VA_ARGS__);              \
-    else                                                   \
-      NLAUNCH(kernel<V + kPlainWalk>, n, st, __VA_ARGS__); \
   } while (0)
 
 constexpr int32_t kInfBits = 0x7f800000;  // +inf: neighbour lists invalid
