@@ -1190,8 +1190,8 @@ struct XsSums {
 // shaders/sph_lambda.glsl:17-77.  Writes dl_s[t] = (max(rho,1e-6), lambda),
 // pred_s[t].w = lambda (read by the delta pass with the position), and the
 // canonical lambda4 row if requested.  The density sum is bit exact
-// (dist2_exact / poly6_accum_exact in the reference's order).
-// V = MODE + 8 * BATCH
+// (dist2_exact, then lam_dens terms added in the reference's order).
+// V = MODE + 8 * BATCH + kQuadWalk
 template <int V>
 __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
                                                 float4* __restrict__ pred_s,
