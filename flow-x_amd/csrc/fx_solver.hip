@@ -729,12 +729,26 @@ enum { VISIT_FULL = 0, VISIT_BUILD = 1, VISIT_LIST = 2 };
 constexpr int VISIT_STREAM = 4;        // one stream per lane; batches run on across row ends
 constexpr int VISIT_MASKED_LOADS = 8;  // with VISIT_STREAM: empty batch slots load nothing
 constexpr int VISIT_ROW_BATCH = 16;    // with VISIT_STREAM: batches stay in a row
-// A scene under kStreamMinParticles (about 8 waves per CU at one lane per
-// particle) leaves most SIMDs with one wave or none, so a pass lasts as long
-// as one wave's instruction stream (~330 cycles per candidate at 25,600
-// particles); such scenes run the quad walk (visit_quad: four lanes per
-// particle), larger ones the stream walks.
-constexpr int kStreamMinParticles = 1 << 17;
+// A small scene leaves most SIMDs with one wave or none, so a pass lasts as
+// long as one wave's instruction stream (~330 cycles per candidate at 25,600
+// particles): the stream walks take about the same time from 26k to 127k
+// particles (lambda 54-58 us).  Small scenes run the quad walk (visit_quad:
+// four lanes per particle), which costs more instructions in all but
+// finishes sooner while SIMDs are idle.  Where it stops paying, per pass
+// (h/2 lattice, ~216 candidates per particle; lambda / delta / XSPH with
+// normals, us, stream vs quad): 26k 54 / 58 / 76 vs 28 / 32 / 65, 65k
+// 56 / 58 / 79 vs 40 / 47 / 101, 90k 56 / 59 / 82 vs 51 / 60 / 141, 127k
+// 58 / 60 / 84 vs 65 / 77 / 175.
+constexpr int kQuadBelowLambda = 104000;
+constexpr int kQuadBelowDelta = 88000;
+constexpr int kQuadBelowXsph = 36000;
+// Kernel template bit: between its kQuadBelow* and kRowBelow, delta and
+// XSPH walk row by row, whose bookkeeping is lighter than the stream walks'
+// (25,600 particles: delta / XSPH 48 / 58 us row by row against 59 / 74 us
+// streamed; 127k: 59 / 81 against 60 / 84).  Lambda's stream walk already
+// batches within rows and stays ahead (127k: 58 us streamed, 64 by row).
+constexpr int kRowWalk = 64;
+constexpr int kRowBelow = 1 << 17;
 
 struct PV {
   float4 p, v;
@@ -859,10 +873,10 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
     }
   }
   if (M == VISIT_FULL) {
-    // row by row: passes without a stream walk (the normal pass), and the
-    // stream walks' lanes with a row past max_scan, which keep FX_BATCH -
-    // this path's registers count against the whole kernel (8 would take
-    // XSPH from 73 to 107 VGPRs)
+    // row by row: mid-size scenes (kRowWalk), passes without a stream walk
+    // (the normal pass), and the stream walks' lanes with a row past
+    // max_scan, which keep FX_BATCH - this path's registers count against
+    // the whole kernel (8 would take XSPH from 73 to 107 VGPRs)
     constexpr int kPlainBatch = (MODE & VISIT_STREAM) ? FX_BATCH : 8;
     // the next row's run bounds are loaded while the current row is scanned
     auto row = [&](int r, int& c0, int& s, int& e) {
@@ -977,9 +991,9 @@ __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restr
 }
 
 // Small scenes, full stencil, four lanes per particle (kernel template bit
-// kQuadWalk).  Under kStreamMinParticles most SIMDs hold one wave or none,
-// and a pass lasts as long as one wave's instruction stream; four lanes per
-// particle quarter each stream and fill four times the SIMDs.  The quad walks
+// kQuadWalk).  In a small scene most SIMDs hold one wave or none, and a pass
+// lasts as long as one wave's instruction stream; four lanes per particle
+// quarter each stream and fill four times the SIMDs.  The quad walks
 // its particle's rows exactly as visit's row-by-row path does (same runs, same
 // per-cell cut past max_scan); in each round lane j takes slot k + j of the
 // run, computes its term (a zero term past the run's end or out of range),
@@ -1230,7 +1244,7 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
       visit_quad(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const LamTerm& tt) { sum.add(tt); });
       if (gt & 3) return;
     } else {
-      visit_auto<MODE | VISIT_STREAM | VISIT_ROW_BATCH>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+      visit_auto<MODE | ((V & kRowWalk) ? 0 : VISIT_STREAM | VISIT_ROW_BATCH)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2_exact(dx, dy, dz);
         if (r2 < h2) {
@@ -1306,7 +1320,7 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
         visit_quad(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const DelTerm& tt) { sum.add(tt); });
         if (gt & 3) return;
       } else {
-        visit_auto<MODE | VISIT_STREAM>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+        visit_auto<MODE | ((V & kRowWalk) ? 0 : VISIT_STREAM)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
           const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
           const float r2 = dist2_exact(dx, dy, dz);
           if (r2 < h2 && r2 > 1e-12f && q != t) sum.add(DelTerm{del_coef(k, li, pj.w, inv_wq, scorr, r2), dx, dy, dz});
@@ -1417,7 +1431,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
       };
       // xs_term and XsSums::add, written out so each sum is updated as soon
       // as its weight is known (fewer live registers; the same operations)
-      visit_auto<MODE | VISIT_STREAM | VISIT_MASKED_LOADS>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
+      visit_auto<MODE | ((V & kRowWalk) ? 0 : VISIT_STREAM | VISIT_MASKED_LOADS)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
 #pragma clang fp contract(off)
         const float4 vj = c.v;
         const float dx = pi.x - c.p.x, dy = pi.y - c.p.y, dz = pi.z - c.p.z;
@@ -1700,12 +1714,15 @@ static int validate(const FxParams* p, const FxState* s, bool need_work) {
 #define NLAUNCH(kernel, n, st, ...) \
   FX_LAUNCH(kernel, dim3(grid_for(n, g_tuning.block)), dim3(g_tuning.block), st, __VA_ARGS__)
 // a full-stencil neighbour pass with the walk chosen by scene size: four
-// lanes per particle under kStreamMinParticles (V unparenthesised: the
-// launch's profile label is the kernel text)
-#define QLAUNCH(kernel, V, n, st, ...)                                                                    \
+// lanes per particle under quad_below, row by row under row_below, else the
+// stream walk (V unparenthesised: the launch's profile label is the kernel
+// text)
+#define QLAUNCH(kernel, V, n, quad_below, row_below, st, ...)                                             \
   do {                                                                                                    \
-    if ((n) >= kStreamMinParticles)                                                                       \
+    if ((n) >= (quad_below) && (n) >= (row_below))                                                        \
       NLAUNCH(kernel<V>, n, st, __VA_ARGS__);                                                             \
+    else if ((n) >= (quad_below))                                                                         \
+      NLAUNCH(kernel<V + kRowWalk>, n, st, __VA_ARGS__);                                                  \
     else                                                                                                  \
       FX_LAUNCH(kernel<V + kQuadWalk>, dim3(grid_for(4LL * (n), g_tuning.block)), dim3(g_tuning.block), st, \
                 __VA_ARGS__);                                                                             \
@@ -1894,7 +1911,7 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
     const bool last = it == iters - 1;
     float4* lam_out = (last && last_in_call && s.lambda) ? (float4*)s.lambda : (float4*)nullptr;
     if (!lists)
-      QLAUNCH(k_lambda, VISIT_FULL, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
+      QLAUNCH(k_lambda, VISIT_FULL, n, kQuadBelowLambda, 0, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else if (it == 0)
       NLAUNCH(k_lambda<VISIT_BUILD>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else
@@ -1904,7 +1921,7 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
               w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
               (float4*)nullptr, L, (const int32_t*)w.skey, Scenes{nullptr, 0});
     else
-      QLAUNCH(k_delta, VISIT_FULL, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
+      QLAUNCH(k_delta, VISIT_FULL, n, kQuadBelowDelta, kRowBelow, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
               w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
               (float4*)nullptr, L, (const int32_t*)nullptr, Scenes{nullptr, 0});
     cur ^= 1;
@@ -1925,11 +1942,11 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
       // resident: by slot for the next substep; last of the call: for the
       // next call; otherwise straight into the caller's buffer
       float4* nrm = (resident_out || last_in_call) ? w.normal_next : (float4*)s.normal;
-      QLAUNCH(k_xsph, 32 + 16 + VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
+      QLAUNCH(k_xsph, 32 + 16 + VISIT_FULL, n, kQuadBelowXsph, kRowBelow, st, n, p, g, k, col, (const float4*)w.pred_s[0],
               (const float4*)w.vel_s, w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr,
               (float4*)nullptr, (float4*)nullptr, L, Scenes{nullptr, 0}, nrm, resident_out ? 1 : 0);
     } else {
-      QLAUNCH(k_xsph, 16 + VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+      QLAUNCH(k_xsph, 16 + VISIT_FULL, n, kQuadBelowXsph, kRowBelow, st, n, p, g, k, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
               w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, (float4*)nullptr,
               L, Scenes{nullptr, 0});
     }
@@ -2036,7 +2053,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
         LAUNCH(k_gather, n, st, n, w.sidx, (const int32_t*)nullptr, (const float4*)s.pred, (const float4*)nullptr,
                (const float4*)nullptr, w.pred_s[0], (float4*)nullptr, (float2*)nullptr,
                (int32_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr);
-        QLAUNCH(k_lambda, VISIT_FULL, n, st, n, g, k, w.pred_s[0], w.off, w.dl_s, w.sidx, (float4*)s.lambda, L0, Scenes{nullptr, 0});
+        QLAUNCH(k_lambda, VISIT_FULL, n, kQuadBelowLambda, 0, st, n, g, k, w.pred_s[0], w.off, w.dl_s, w.sidx, (float4*)s.lambda, L0, Scenes{nullptr, 0});
       }
       break;
     case FX_PASS_DELTA:
@@ -2050,7 +2067,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
                (const float4*)s.lambda, w.pred_s[0], (float4*)nullptr, w.dl_s, (int32_t*)nullptr,
                (int32_t*)nullptr, (int32_t*)nullptr);
         LAUNCH(k_w_from_lambda, n, st, n, w.pred_s[0], (const float2*)w.dl_s);
-        QLAUNCH(k_delta, VISIT_FULL, n, st, n, g, k, (const float4*)w.pred_s[0], (const float2*)w.dl_s, w.off,
+        QLAUNCH(k_delta, VISIT_FULL, n, kQuadBelowDelta, kRowBelow, st, n, g, k, (const float4*)w.pred_s[0], (const float2*)w.dl_s, w.off,
                 (float4*)nullptr, (const float4*)nullptr, (float4*)nullptr, w.sidx, (float4*)s.delta, L0,
                 (const int32_t*)nullptr, Scenes{nullptr, 0});
       }
@@ -2075,7 +2092,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
       if (n > 0) {
         LAUNCH(k_gather_xsph, n, st, n, w.sidx, (const float4*)s.pred, (const float4*)s.vel,
                (const float4*)s.lambda, w.pred_s[0], w.vel_s);
-        QLAUNCH(k_xsph, VISIT_FULL, n, st, n, p, g, k, col, (const float4*)w.pred_s[0],
+        QLAUNCH(k_xsph, VISIT_FULL, n, kQuadBelowXsph, kRowBelow, st, n, p, g, k, col, (const float4*)w.pred_s[0],
                 (const float4*)w.vel_s, w.off, w.sidx, (float4*)s.delta, (float4*)nullptr,
                 (float4*)nullptr, (float4*)nullptr, (float4*)nullptr, L0, Scenes{nullptr, 0});
       }
