@@ -368,20 +368,30 @@ def adjoint_workload(fx, substeps, directions, fd_checks=((4, 1e-4, 1e-8), (8, 1
             v = v0.clone().requires_grad_(True)
             rollout(d, x0, v, g, tr, 2, [], 1)  # warm-up
             # the per-substep records are allocated ahead: the timed region
-            # holds the substeps, not the allocator
+            # holds the substeps, not the allocator.  The host drives every
+            # substep, so wall times vary from run to run: the pair is timed
+            # three times and the medians are reported
             records = [d.new_decisions() for _ in range(substeps)]
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            x, _ = rollout(d, x0, v, g, tr, substeps, records, 1, check_overflow=False)
-            L = loss_fn(d, x, target, com_t)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            L.backward()
-            torch.cuda.synchronize()
-            t2 = time.perf_counter()
-            out.update(particles=len(x0), substeps=substeps, forward_s=round(t1 - t0, 4),
-                       backward_s=round(t2 - t1, 4), adjoint_over_forward=round((t2 - t1) / (t1 - t0), 3),
-                       list_overflow=d.overflowed(), fp32_grad_v0_norm=float(v.grad.norm()))
+            fwd, bwd = [], []
+            for _ in range(3):
+                v = v0.clone().requires_grad_(True)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                x, _ = rollout(d, x0, v, g, tr, substeps, records, 1, check_overflow=False)
+                L = loss_fn(d, x, target, com_t)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                L.backward()
+                torch.cuda.synchronize()
+                t2 = time.perf_counter()
+                fwd.append(t1 - t0)
+                bwd.append(t2 - t1)
+                g.grad = tr.grad = None
+            tf, tb = float(np.median(fwd)), float(np.median(bwd))
+            out.update(particles=len(x0), substeps=substeps, forward_s=round(tf, 4), backward_s=round(tb, 4),
+                       adjoint_over_forward=round(tb / tf, 3), forward_s_runs=[round(t, 4) for t in fwd],
+                       backward_s_runs=[round(t, 4) for t in bwd], list_overflow=d.overflowed(),
+                       fp32_grad_v0_norm=float(v.grad.norm()))
             del x, L, v
             torch.cuda.empty_cache()
             continue
