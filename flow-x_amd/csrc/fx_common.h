@@ -114,11 +114,19 @@ __device__ __forceinline__ float dist2_exact(float dx, float dy, float dz) {
   return dx * dx + dy * dy + dz * dz;
 }
 
-// acc + mass * W_poly6 with W = ((coef * d) * d) * d, d = h^2 - r^2
-// (shaders/sph_common.glsl:84-91, shaders/sph_lambda.glsl:57).
-__device__ __forceinline__ float poly6_accum_exact(float acc, float mass, float coef, float d) {
+// mass * W_poly6 with W = ((coef * d) * d) * d, d = h^2 - r^2
+// (shaders/sph_common.glsl:84-91, shaders/sph_lambda.glsl:57), and its
+// addition to a running sum, each rounded as written.
+__device__ __forceinline__ float poly6_term_exact(float mass, float coef, float d) {
 #pragma clang fp contract(off)
-  return acc + mass * (((coef * d) * d) * d);
+  return mass * (((coef * d) * d) * d);
+}
+__device__ __forceinline__ float add_exact(float acc, float t) {
+#pragma clang fp contract(off)
+  return acc + t;
+}
+__device__ __forceinline__ float poly6_accum_exact(float acc, float mass, float coef, float d) {
+  return add_exact(acc, poly6_term_exact(mass, coef, d));
 }
 
 // ---- neighbour iteration ----------------------------------------------------
@@ -182,6 +190,43 @@ __device__ __forceinline__ void scan_run(int s, int e, LD&& ld, F&& f, RF&& refi
     auto d = ld(q);
     refine(q, d);
     f(q, d);
+  }
+}
+
+// ---- four lanes per item ----------------------------------------------------
+// Lane J's value of v, read by every lane of its quad (DPP quad_perm).  All
+// four lanes of the quad must be active.
+template <int J>
+__device__ __forceinline__ float quad_lane(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), J * 0x55, 0xF, 0xF,
+                                                              false));
+}
+
+// Slots [s, e) shared by the four lanes of a quad (lane j of 0..3): in each
+// round lane j takes slot k + j, ld(q) loads it (q clamped into the run),
+// tm(q, data, valid) makes its term, and fold(term) is called with the four
+// lanes' terms in slot order on every lane (term.lane<J>() reads lane J's).
+// B rounds of loads are in flight per lane.  The quad must run this with the
+// same s, e.
+template <int B, class LD, class TM, class FD>
+__device__ __forceinline__ void quad_run(int s, int e, int j, LD&& ld, TM&& tm, FD&& fold) {
+#pragma unroll 1
+  for (int k = s; k < e; k += 4 * B) {
+    decltype(ld(0)) d[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) d[u] = ld(min(k + 4 * u + j, e - 1));
+    decltype(tm(0, d[0], false)) tt[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) tt[u] = tm(k + 4 * u + j, d[u], k + 4 * u + j < e);
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      if (k + 4 * u < e) {  // the same for the four lanes
+        fold(tt[u].template lane<0>());
+        fold(tt[u].template lane<1>());
+        fold(tt[u].template lane<2>());
+        fold(tt[u].template lane<3>());
+      }
+    }
   }
 }
 

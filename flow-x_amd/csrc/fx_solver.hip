@@ -1010,37 +1010,12 @@ __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restr
 constexpr int kQuadWalk = 128;
 constexpr int kQuadBatch = 4;  // rounds of loads in flight per lane
 
-template <int J>
-__device__ __forceinline__ float quad_lane(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), J * 0x55, 0xF, 0xF,
-                                                              false));
-}
-
 template <class LD, class TM, class FD>
 __device__ __forceinline__ void visit_quad(const Grid& g, const int32_t* __restrict__ off, int max_scan, int bx,
                                            int by, int bz, int j, LD&& ld, TM&& tm, FD&& fold) {
   constexpr int B = kQuadBatch;
   const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
-  auto run = [&](int s, int e) {
-#pragma unroll 1
-    for (int k = s; k < e; k += 4 * B) {
-      decltype(ld(0)) d[B];
-#pragma unroll
-      for (int u = 0; u < B; ++u) d[u] = ld(min(k + 4 * u + j, e - 1));
-      decltype(tm(0, d[0], false)) tt[B];
-#pragma unroll
-      for (int u = 0; u < B; ++u) tt[u] = tm(k + 4 * u + j, d[u], k + 4 * u + j < e);
-#pragma unroll
-      for (int u = 0; u < B; ++u) {
-        if (k + 4 * u < e) {  // the same for the four lanes
-          fold(tt[u].template lane<0>());
-          fold(tt[u].template lane<1>());
-          fold(tt[u].template lane<2>());
-          fold(tt[u].template lane<3>());
-        }
-      }
-    }
-  };
+  auto run = [&](int s, int e) { quad_run<B>(s, e, j, ld, tm, fold); };
   auto row = [&](int r, int& c0, int& s, int& e) {
     const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
     if (z < 0 || z >= g.cz || y < 0 || y >= g.cy) {

@@ -23,6 +23,10 @@
 
 namespace fx {
 
+// splat threads (groups of FX_SPLAT_M samples) below which each thread's
+// samples get four lanes (a small lattice leaves most SIMDs idle)
+constexpr long long kSplatQuadBelow = 1 << 19;
+
 // ---- marching-cubes tables, derived at compile time -------------------------
 struct McTables {
   int8_t tri[256][16];  // edge triples, -1 terminated
@@ -164,16 +168,36 @@ __device__ __forceinline__ float axis_gap(float p, float lo, float hi) {
 // cells of its own, shaders/surface_splat.glsl:62-82) in the reference's
 // order - cells by increasing x, slots ascending, at most max_scan per cell -
 // so every sum is the one-sample kernel's, bit for bit.
+// The M samples' density terms of one candidate (zero where it does not
+// contribute), for the four-lane form of the splat.
 template <int M>
+struct SplatTerm {
+  float t[M];
+  template <int J>
+  __device__ __forceinline__ SplatTerm lane() const {
+    SplatTerm o;
+#pragma unroll
+    for (int u = 0; u < M; ++u) o.t[u] = quad_lane<J>(t[u]);
+    return o;
+  }
+};
+
+// MQ = M samples per thread, + kSplatQuad for four lanes per thread
+constexpr int kSplatQuad = 16;
+template <int MQ>
 __global__ __launch_bounds__(256) void k_splat_m(FxSurface sf, Grid g, float mass, float rho0, float h2,
                                                  float coef, int max_scan, Collider col,
                                                  const float4* __restrict__ pos_s,
                                                  const int32_t* __restrict__ off,
                                                  const int32_t* __restrict__ dcell_bits, float kr,
                                                  float* __restrict__ field) {
+  constexpr int M = MQ % kSplatQuad;
+  constexpr bool QUAD = MQ >= kSplatQuad;  // four lanes per thread's samples
   const int nx = sf.dims[0], ny = sf.dims[1], nz = sf.dims[2];
   const int groups = (nx + M - 1) / M;
-  const long long q0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long q0 = QUAD ? gt >> 2 : gt;
+  const int jq = (int)(gt & 3);
   if (q0 >= (long long)groups * ny * nz) return;
   const int gx0 = (int)(q0 % groups) * M, gy = (int)((q0 / groups) % ny), gz = (int)(q0 / ((long long)groups * ny));
   const int m = min(M, nx - gx0);
@@ -239,7 +263,32 @@ __global__ __launch_bounds__(256) void k_splat_m(FxSurface sf, Grid g, float mas
         }
       };
       auto ld = [&](int q) { return pos_s[q]; };
-      if (E - A <= max_scan) {
+      if constexpr (QUAD) {
+        // the quad splits each run's candidates and folds the terms back in
+        // slot order (quad_run): the same sums as `body`, bit for bit
+        auto tm = [&](int q, const float4& pj, bool valid) {
+          SplatTerm<M> o;
+#pragma unroll
+          for (int u = 0; u < M; ++u) {
+            const float dx = px[u] - pj.x, dy2 = py - pj.y, dz2 = pz - pj.z;
+            const float r2 = dist2_exact(dx, dy2, dz2);
+            o.t[u] = (valid && q >= slo[u] && q < shi[u] && r2 < h2) ? poly6_term_exact(mass, coef, h2 - r2) : 0.0f;
+          }
+          return o;
+        };
+        auto fold = [&](const SplatTerm<M>& tt) {
+#pragma unroll
+          for (int u = 0; u < M; ++u) acc[u] = add_exact(acc[u], tt.t[u]);
+        };
+        if (E - A <= max_scan) {
+          quad_run<4>(A, E, jq, ld, tm, fold);
+        } else {
+          for (int x = ua; x <= ub; ++x) {
+            const int cs = off[row + x], ce = min(off[row + x + 1], cs + max_scan);
+            quad_run<4>(cs, ce, jq, ld, tm, fold);
+          }
+        }
+      } else if (E - A <= max_scan) {
         scan_run(A, E, ld, body);  // no cell of the run reaches the per-cell cap
       } else {
         for (int x = ua; x <= ub; ++x) {
@@ -249,6 +298,7 @@ __global__ __launch_bounds__(256) void k_splat_m(FxSurface sf, Grid g, float mas
       }
     }
   }
+  if (QUAD && jq != 0) return;
 #pragma unroll
   for (int u = 0; u < M; ++u) {
     if (u >= m) break;
@@ -752,9 +802,15 @@ int fx_splat(const FxParams* p, const FxSurface* surf, const FxState* s, float* 
   Collider col = make_collider(*p, s->collider);
   constexpr int M = FX_SPLAT_M;
   const long long threads = (long long)((surf->dims[0] + M - 1) / M) * surf->dims[1] * surf->dims[2];
-  FX_LAUNCH(k_splat_m<M>, dim3(grid_for(threads)), dim3(256), st, *surf, make_grid(*p), p->mass, p->rest_density,
-            kr * kr, coef, p->max_cell_scan > 0 ? p->max_cell_scan : 512, col, (const float4*)pos_s, off,
-            (const int32_t*)dmax, kr, field);
+  // a small lattice leaves SIMDs idle: four lanes per thread's samples
+  if (threads < kSplatQuadBelow)
+    FX_LAUNCH(k_splat_m<M + kSplatQuad>, dim3(grid_for(4 * threads)), dim3(256), st, *surf, make_grid(*p), p->mass,
+              p->rest_density, kr * kr, coef, p->max_cell_scan > 0 ? p->max_cell_scan : 512, col,
+              (const float4*)pos_s, off, (const int32_t*)dmax, kr, field);
+  else
+    FX_LAUNCH(k_splat_m<M>, dim3(grid_for(threads)), dim3(256), st, *surf, make_grid(*p), p->mass,
+              p->rest_density, kr * kr, coef, p->max_cell_scan > 0 ? p->max_cell_scan : 512, col,
+              (const float4*)pos_s, off, (const int32_t*)dmax, kr, field);
   (void)S;
   return check_launch("fx_splat");
 }
