@@ -24,7 +24,8 @@
 namespace fx {
 
 // splat threads (groups of FX_SPLAT_M samples) below which each thread's
-// samples get four lanes (a small lattice leaves most SIMDs idle)
+// samples get four lanes (a small lattice leaves most SIMDs idle); the two
+// forms cross near 530k threads (lattice side ~102)
 constexpr long long kSplatQuadBelow = 1 << 19;
 
 // ---- marching-cubes tables, derived at compile time -------------------------
