@@ -45,6 +45,37 @@ def test_batch_equals_plain_per_scene(fx):
         assert np.array_equal(batch.scene_velocities(i), s.read_velocities()), f"scene {i} velocities"
 
 
+@pytest.mark.parametrize("lattice", [(50, 50, 20), (70, 68, 20), (80, 80, 32)])
+def test_walks_by_scene_size_are_bit_identical(fx, lattice):
+    """A plain scene picks its neighbour walk by size (four lanes per particle
+    below a pass's threshold, row by row up to 2^17, the stream walks above);
+    the batched launch always streams one lane per particle.  A one-scene
+    batch must therefore equal the plain run bit for bit: at 50,000 particles
+    lambda and delta walk in quads and XSPH row by row, at 95,200 lambda in
+    quads and delta and XSPH row by row, at 204,800 everything streams.
+    Cohesion on (the XSPH pass also computes the normals) and a collider."""
+    import torch
+
+    from flowx_amd.config import converging_update
+
+    def scene():
+        c = converging_update(fx.lattice_config((-1.0, -1.0, 0.0), (1.0, 1.0, 2.0), 0.025, lattice, iterations=4,
+                                                viscosity=0.01, surface_tension=0.001))
+        c.extra.update(colliders=[("box", (-0.2, -0.2, 0.0), (0.2, 0.2, 0.3))], collider_resolution=40)
+        return c
+
+    steps = 5
+    batch = fx.BatchSolver([scene()])
+    for _ in range(steps):
+        batch.substep()
+    s = fx.Solver(scene(), keep_scratch=False)
+    s.reset()
+    s.substeps_n(batch.dts[0], steps)
+    torch.cuda.synchronize()
+    assert np.array_equal(batch.scene_positions(0), s.read_positions())
+    assert np.array_equal(batch.scene_velocities(0), s.read_velocities())
+
+
 def test_batch_from_generated_scene_set_runs(fx):
     import torch
 
