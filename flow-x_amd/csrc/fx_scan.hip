@@ -103,8 +103,49 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const int32_t* __re
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partial[nb];
 }
 
+// Up to kSmallScanMax items, one block scans everything in one launch (each
+// thread kSmallScanItems consecutive items): a small scene's cell offsets
+// then cost one launch instead of three.
+constexpr int kSmallScanItems = 16;
+constexpr long long kSmallScanMax = 1024LL * kSmallScanItems;
+
+__global__ __launch_bounds__(1024) void k_scan_small(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                     int n) {
+  __shared__ int lds[16];
+  const int per = (n + 1023) / 1024;  // <= kSmallScanItems
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int begin = threadIdx.x * per;
+  int v[kSmallScanItems];
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < kSmallScanItems; ++i) {
+    v[i] = (i < per && begin + i < n) ? in[begin + i] : 0;
+    s += v[i];
+  }
+  int incl = s;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += u;
+  }
+  if (lane == 63) lds[wave] = incl;
+  __syncthreads();
+  for (int w = 0; w < wave; ++w) incl += lds[w];
+  int run = incl - s;
+#pragma unroll
+  for (int i = 0; i < kSmallScanItems; ++i) {
+    if (i < per && begin + i < n) out[begin + i] = run;
+    run += v[i];
+  }
+  if (threadIdx.x == 1023) out[n] = incl;
+}
+
 int exclusive_scan_i32(const int32_t* counts, int32_t* out, long long n, int32_t* partial,
                        hipStream_t st) {
+  if (n <= kSmallScanMax) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, st, counts, out, (int)n);
+    return check_launch("exclusive_scan_i32");
+  }
   long long nb = (n + kScanTile - 1) / kScanTile;
   if (nb < 1) nb = 1;
   if (nb > kMaxPartials) return fail(FX_ERANGE, "scan of %lld items exceeds %d tiles", n, kMaxPartials);
