@@ -1004,9 +1004,9 @@ __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restr
 // factor of a fused multiply-add) equals skipping the candidate: the sums
 // start at +0 and round to nearest, so they are never -0, and x + (+-0) = x.
 // So the four copies and the one-lane walks agree bit for bit.  tm(q, data,
-// valid) returns the
-// term of slot q; fold(term) adds one term.  All four lanes of a quad run the
-// same control flow (same cell, same runs), which the DPP reads need.
+// valid) returns the term of slot q; fold(term) adds one term.  All four
+// lanes of a quad run the same control flow (same cell, same runs), which
+// the DPP reads need.
 constexpr int kQuadWalk = 128;
 constexpr int kQuadBatch = 4;  // rounds of loads in flight per lane
 
@@ -1180,7 +1180,7 @@ struct XsSums {
 // pred_s[t].w = lambda (read by the delta pass with the position), and the
 // canonical lambda4 row if requested.  The density sum is bit exact
 // (dist2_exact, then lam_dens terms added in the reference's order).
-// V = MODE + 8 * BATCH + kQuadWalk
+// V = MODE + 8 (batched scenes) + kRowWalk or kQuadWalk
 template <int V>
 __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
                                                 float4* __restrict__ pred_s,
