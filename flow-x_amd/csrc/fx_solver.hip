@@ -797,7 +797,50 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
       const int c0 = in ? (z * g.cy + y) * g.cx + x0 : 0;
       S[r] = in ? off[c0] : 0;
       E[r] = in ? off[c0 + (x1 - x0) + 1] : 0;
-      big |= E[r] - S[r] > max_scan;
+      big |= E[r] - S[r] > max_scan || E[r] - S[r] > 0xFFFF;
+    }
+    if (!big && (MODE & VISIT_MASKED_LOADS)) {
+      // as below, with the queue's row lengths held two to a register (each
+      // is at most 0xFFFF): four registers fewer, which keeps the XSPH pass
+      // at 7 waves per SIMD instead of 6
+      uint32_t Lp[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+        Lp[i] = (uint32_t)(E[2 * i] - S[2 * i]) | (2 * i + 1 < 9 ? (uint32_t)(E[2 * i + 1] - S[2 * i + 1]) << 16 : 0u);
+      int k = S[0], e = S[0] + (int)(Lp[0] & 0xFFFFu), left = 8;
+#pragma unroll 1
+      for (;;) {
+        int q[B];
+        bool v[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          while (k >= e && left > 0) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) S[r] = S[r + 1];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) Lp[i] = (Lp[i] >> 16) | (Lp[i + 1] << 16);
+            Lp[4] >>= 16;
+            k = S[0];
+            e = S[0] + (int)(Lp[0] & 0xFFFFu);
+            --left;
+          }
+          v[u] = k < e;
+          q[u] = k;
+          k += v[u] ? 1 : 0;
+        }
+        if (!v[0]) break;
+        decltype(ld(0)) d[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+          if (v[u]) d[u] = ld(q[u]);
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+          if (v[u]) rf(q[u], d[u]);
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+          if (v[u]) f(q[u], d[u]);
+      }
+      return;
     }
     if (!big && (MODE & VISIT_ROW_BATCH)) {
       int k = S[0], e = E[0], left = 8;
