@@ -729,6 +729,7 @@ enum { VISIT_FULL = 0, VISIT_BUILD = 1, VISIT_LIST = 2 };
 constexpr int VISIT_STREAM = 4;        // one stream per lane; batches run on across row ends
 constexpr int VISIT_MASKED_LOADS = 8;  // with VISIT_STREAM: empty batch slots load nothing
 constexpr int VISIT_ROW_BATCH = 16;    // with VISIT_STREAM: batches stay in a row
+constexpr int VISIT_PACKED = 32;       // with VISIT_ROW_BATCH: row lengths two to a register
 // A small scene leaves most SIMDs with one wave or none, so a pass lasts as
 // long as one wave's instruction stream (~330 cycles per candidate at 25,600
 // particles): the stream walks take about the same time from 26k to 127k
@@ -799,10 +800,10 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
       E[r] = in ? off[c0 + (x1 - x0) + 1] : 0;
       big |= E[r] - S[r] > max_scan || E[r] - S[r] > 0xFFFF;
     }
-    if (!big && (MODE & VISIT_MASKED_LOADS)) {
-      // as below, with the queue's row lengths held two to a register (each
-      // is at most 0xFFFF): four registers fewer, which keeps the XSPH pass
-      // at 7 waves per SIMD instead of 6
+    if (!big && !(MODE & VISIT_ROW_BATCH)) {
+      // the queue's row lengths held two to a register (each is at most
+      // 0xFFFF): registers fewer, which keeps the XSPH pass at 7 waves per
+      // SIMD instead of 6
       uint32_t Lp[5];
 #pragma unroll
       for (int i = 0; i < 5; ++i)
@@ -830,9 +831,14 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
         }
         if (!v[0]) break;
         decltype(ld(0)) d[B];
+        if (MODE & VISIT_MASKED_LOADS) {
 #pragma unroll
-        for (int u = 0; u < B; ++u)
-          if (v[u]) d[u] = ld(q[u]);
+          for (int u = 0; u < B; ++u)
+            if (v[u]) d[u] = ld(q[u]);
+        } else {
+#pragma unroll
+          for (int u = 0; u < B; ++u) d[u] = ld(v[u] ? q[u] : q[0]);
+        }
 #pragma unroll
         for (int u = 0; u < B; ++u)
           if (v[u]) rf(q[u], d[u]);
@@ -843,16 +849,30 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
       return;
     }
     if (!big && (MODE & VISIT_ROW_BATCH)) {
-      int k = S[0], e = E[0], left = 8;
+      // VISIT_PACKED: the row lengths two to a register, as below (the
+      // batched lambda pass: 65 -> 62 VGPRs, 8 waves per SIMD; the plain one
+      // is at 8 already and only pays the shifts)
+      constexpr bool P = (MODE & VISIT_PACKED) != 0;
+      uint32_t Lp[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+        Lp[i] = P ? (uint32_t)(E[2 * i] - S[2 * i]) | (2 * i + 1 < 9 ? (uint32_t)(E[2 * i + 1] - S[2 * i + 1]) << 16 : 0u)
+                  : 0u;
+      int k = S[0], e = P ? S[0] + (int)(Lp[0] & 0xFFFFu) : E[0], left = 8;
       auto next_row = [&]() {
         while (k >= e && left > 0) {
 #pragma unroll
           for (int r = 0; r < 8; ++r) {
             S[r] = S[r + 1];
-            E[r] = E[r + 1];
+            if (!P) E[r] = E[r + 1];
+          }
+          if (P) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) Lp[i] = (Lp[i] >> 16) | (Lp[i + 1] << 16);
+            Lp[4] >>= 16;
           }
           k = S[0];
-          e = E[0];
+          e = P ? S[0] + (int)(Lp[0] & 0xFFFFu) : E[0];
           --left;
         }
       };
@@ -870,47 +890,6 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
           if (k + u < e) f(k + u, d[u]);
         k += B;
         next_row();
-      }
-      return;
-    }
-    if (!big) {
-      int k = S[0], e = E[0], left = 8;
-#pragma unroll 1
-      for (;;) {
-        int q[B];
-        bool v[B];
-#pragma unroll
-        for (int u = 0; u < B; ++u) {
-          while (k >= e && left > 0) {
-#pragma unroll
-            for (int r = 0; r < 8; ++r) {
-              S[r] = S[r + 1];
-              E[r] = E[r + 1];
-            }
-            k = S[0];
-            e = E[0];
-            --left;
-          }
-          v[u] = k < e;
-          q[u] = k;
-          k += v[u] ? 1 : 0;
-        }
-        if (!v[0]) break;
-        decltype(ld(0)) d[B];
-        if (MODE & VISIT_MASKED_LOADS) {
-#pragma unroll
-          for (int u = 0; u < B; ++u)
-            if (v[u]) d[u] = ld(q[u]);
-        } else {
-#pragma unroll
-          for (int u = 0; u < B; ++u) d[u] = ld(v[u] ? q[u] : q[0]);
-        }
-#pragma unroll
-        for (int u = 0; u < B; ++u)
-          if (v[u]) rf(q[u], d[u]);
-#pragma unroll
-        for (int u = 0; u < B; ++u)
-          if (v[u]) f(q[u], d[u]);
       }
       return;
     }
@@ -1027,7 +1006,7 @@ __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restr
     if (ok)
       visit<VISIT_LIST>(g, off, max_scan, bx, by, bz, t, n, L, ld, f, rf);
     else
-      visit<VISIT_FULL | (MODE & (VISIT_STREAM | VISIT_MASKED_LOADS | VISIT_ROW_BATCH))>(g, off, max_scan, bx, by, bz, t, n, L, ld, f, rf);
+      visit<VISIT_FULL | (MODE & (VISIT_STREAM | VISIT_MASKED_LOADS | VISIT_ROW_BATCH | VISIT_PACKED))>(g, off, max_scan, bx, by, bz, t, n, L, ld, f, rf);
   } else {
     visit<MODE>(g, off, max_scan, bx, by, bz, t, n, L, ld, f, rf);
   }
@@ -1262,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
       visit_quad(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const LamTerm& tt) { sum.add(tt); });
       if (gt & 3) return;
     } else {
-      visit_auto<MODE | ((V & kRowWalk) ? 0 : VISIT_STREAM | VISIT_ROW_BATCH)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+      visit_auto<MODE | ((V & kRowWalk) ? 0 : VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0))>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2_exact(dx, dy, dz);
         if (r2 < h2) {
