@@ -1490,16 +1490,25 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
 // position / velocity (and correction / prediction when asked) and pos_s.
 // Keeping the collider search out of the neighbour pass keeps that pass's
 // register count, and so its occupancy, down (63 instead of 81 VGPRs).
-__global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p, Collider col,
+// V = 8: batched scenes (each particle's params and collider from its scene).
+template <int V>
+__global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p_, Collider col_,
                                                     const float4* __restrict__ pred_s,
                                                     const float4* __restrict__ vel_s,
                                                     const float4* __restrict__ dv_s,
                                                     const int32_t* __restrict__ sidx,
                                                     float4* __restrict__ delta4, float4* __restrict__ pos4,
                                                     float4* __restrict__ vel4, float4* __restrict__ pos_s,
-                                                    float4* __restrict__ pred4_out) {
+                                                    float4* __restrict__ pred4_out, Scenes S) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  FxParams p = p_;
+  Collider col = col_;
+  if (V & 8) {
+    const SceneDev& sc = S.s[scene_of(S, t)];
+    p = sc.p;
+    col = sc.col;
+  }
   const float4 pi = pred_s[t], vi = vel_s[t], dv = dv_s[t];
   const int i = min(max(sidx[t], 0), n - 1);
   if (delta4) delta4[i] = make_float4(dv.x, dv.y, dv.z, 0.0f);
@@ -1956,8 +1965,9 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
              (const int32_t*)w.sidx, w.pos_r, w.vel_r, w.key, w.cnt, w.rank,
              (cohesion && next_is_last) ? (float4*)s.normal : (float4*)nullptr);
     } else
-      LAUNCH(k_finalize_s, n, st, n, p, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
-             (const float4*)w.pred_s[1], w.sidx, delta_out, (float4*)s.pos, (float4*)s.vel, pos_s_out, pred_out);
+      LAUNCH(k_finalize_s<0>, n, st, n, p, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+             (const float4*)w.pred_s[1], w.sidx, delta_out, (float4*)s.pos, (float4*)s.vel, pos_s_out, pred_out,
+             Scenes{nullptr, 0});
     if (fuse_normals && !resident_out) mark_normals(s.work, s.normal, p, !last_in_call);
   }
   return check_launch("substep");
@@ -2239,9 +2249,14 @@ static int batch_step(const FxScene* scenes, int count, float* pos, float* vel, 
     cur ^= 1;
   }
   if (iters == 0) LAUNCH(k_velocity_s, n, st, n, scenes[0].params.dt, w.pred_s[0], w.pos_s, w.dl_s, w.vel_s);
-  NLAUNCH(k_xsph<8 + 4 + VISIT_FULL>, n, st, n, FxParams{}, Grid{}, Consts{}, Collider{}, (const float4*)w.pred_s[0],
-          (const float4*)w.vel_s, w.off, w.sidx, (float4*)nullptr, (float4*)pos, (float4*)vel, w.pos_s,
+  // XSPH leaves dv by slot (in pred_s[1], free after the iterations) and the
+  // finalize runs on its own, as in the plain substep: without the collider
+  // search the neighbour pass holds fewer registers (91 -> 73 VGPRs)
+  NLAUNCH(k_xsph<8 + 16 + VISIT_FULL>, n, st, n, FxParams{}, Grid{}, Consts{}, Collider{}, (const float4*)w.pred_s[0],
+          (const float4*)w.vel_s, w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr, (float4*)nullptr,
           (float4*)nullptr, L0, S);
+  LAUNCH(k_finalize_s<8>, n, st, n, FxParams{}, Collider{}, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
+         (const float4*)w.pred_s[1], w.sidx, (float4*)nullptr, (float4*)pos, (float4*)vel, w.pos_s, (float4*)nullptr, S);
   return check_launch("batch");
 }
 }  // namespace fx
