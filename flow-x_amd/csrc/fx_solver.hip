@@ -1030,12 +1030,15 @@ __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restr
 // lanes of a quad run the same control flow (same cell, same runs), which
 // the DPP reads need.
 constexpr int kQuadWalk = 128;
-constexpr int kQuadBatch = 4;  // rounds of loads in flight per lane
 
-template <class LD, class TM, class FD>
+// B: rounds of loads in flight per lane (dam break, 25,600 particles, B = 2 /
+// 4 / 8: lambda 29.9 / 25.3 / 23.9 us, delta 30.5 / 28.7 / 28.2, XSPH 43.0 /
+// 44.0 / 47.8; at 65k-90k particles B = 8 costs delta 2% and lambda up to 2%,
+// its registers taking the quad kernels from 8 to 6 waves per SIMD): lambda
+// takes 8, delta and XSPH 4
+template <int B, class LD, class TM, class FD>
 __device__ __forceinline__ void visit_quad(const Grid& g, const int32_t* __restrict__ off, int max_scan, int bx,
                                            int by, int bz, int j, LD&& ld, TM&& tm, FD&& fold) {
-  constexpr int B = kQuadBatch;
   const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
   auto run = [&](int s, int e) { quad_run<B>(s, e, j, ld, tm, fold); };
   auto row = [&](int r, int& c0, int& s, int& e) {
@@ -1238,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
         const bool in = valid && r2 < h2, grad = in && q != t && r2 > 1e-12f;
         return lam_grad(in ? lam_dens(k, r2) : 0.0f, grad ? lam_coef(k, r2) : 0.0f, dx, dy, dz);
       };
-      visit_quad(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const LamTerm& tt) { sum.add(tt); });
+      visit_quad<8>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const LamTerm& tt) { sum.add(tt); });
       if (gt & 3) return;
     } else {
       visit_auto<MODE | ((V & kRowWalk) ? 0 : VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0))>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
@@ -1314,7 +1317,7 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
           const bool in = valid && r2 < h2 && r2 > 1e-12f && q != t;
           return DelTerm{in ? del_coef(k, li, pj.w, inv_wq, scorr, r2) : 0.0f, dx, dy, dz};
         };
-        visit_quad(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const DelTerm& tt) { sum.add(tt); });
+        visit_quad<4>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const DelTerm& tt) { sum.add(tt); });
         if (gt & 3) return;
       } else {
         visit_auto<MODE | ((V & kRowWalk) ? 0 : VISIT_STREAM)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
@@ -1412,7 +1415,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
         const XsTerm z{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         return valid && r2 < h2 && q != t ? xs_term<NORMAL>(k, vi, vj, dx, dy, dz, r2) : z;
       };
-      visit_quad(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const XsTerm& tt) { sum.add(tt); });
+      visit_quad<4>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const XsTerm& tt) { sum.add(tt); });
       if (gt & 3) return;
     } else {
       // stage 1 loads positions; stage 2 loads velocities only for candidates
