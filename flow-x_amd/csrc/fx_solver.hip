@@ -479,11 +479,23 @@ __global__ __launch_bounds__(256) void k_finalize_predict(int n, FxParams p, Col
   if (nrm_canon) nrm_canon[min(max(sidx[t], 0), n - 1)] = nrm;
 }
 
-// Counting-sort scatter of (particle << 32 | previous slot) pairs.
+// Counting-sort scatter of (particle << 32 | previous slot) pairs.  The
+// counts are dead once scanned, so the same launch (over max(n, n_cnt)
+// threads) zeroes them for the next substep's counting, clears the big-cell
+// queue's count (flags[0]) for the canonicalisation that follows and marks
+// the neighbour lists invalid (flags[2] = kInfBits): three fills fewer.
+constexpr int32_t kInfBits = 0x7f800000;  // +inf: neighbour lists invalid
+
 __global__ void k_scatter_pairs(int n, const int32_t* __restrict__ key, const int32_t* __restrict__ rank,
                                 const int32_t* __restrict__ off, const int32_t* __restrict__ pid,
-                                uint64_t* __restrict__ pairs) {
+                                uint64_t* __restrict__ pairs, int32_t* __restrict__ cnt, int n_cnt,
+                                int32_t* __restrict__ flags) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n_cnt) cnt[t] = 0;
+  if (t == 0) {
+    flags[0] = 0;
+    flags[2] = kInfBits;
+  }
   if (t >= n) return;
   pairs[off[key[t]] + rank[t]] = ((uint64_t)(uint32_t)pid[t] << 32) | (uint32_t)t;
 }
@@ -1737,8 +1749,6 @@ static int validate(const FxParams* p, const FxState* s, bool need_work) {
                 __VA_ARGS__);                                                                             \
   } while (0)
 
-constexpr int32_t kInfBits = 0x7f800000;  // +inf: neighbour lists invalid
-
 static Lists make_lists(const Ws& w, const Consts& k, bool enabled) {
   Lists L;
   L.mask = enabled ? w.mask : nullptr;
@@ -1775,11 +1785,11 @@ int sort_queued_runs(const int32_t* off, int32_t* data, const int32_t* queue, co
 }
 
 // The same for the (particle << 32 | previous slot) pairs of a resident
-// grid build: ordering the pairs orders the particles.
+// grid build: ordering the pairs orders the particles.  big_count must be
+// zero on entry (k_scatter_pairs clears it).
 static int canon_pairs(const int32_t* off, uint64_t* pairs, int n_cells, int32_t* big_list, int32_t* big_count,
                        hipStream_t st) {
   if (n_cells <= 0) return FX_OK;
-  FX_TRY(fill(big_count, 1, 0, st));
   LAUNCH(k_canon_cells<uint64_t>, n_cells, st, n_cells, off, pairs, big_list, big_count);
   FX_LAUNCH(k_sort_big_cells<uint64_t>, dim3(kBigCellBlocks), dim3(256), st, off, pairs, (const int32_t*)big_list,
             (const int32_t*)big_count);
@@ -1844,15 +1854,15 @@ static int build_grid_resident(const FxParams& p, FxState& s, const Ws& w, int t
   const float4* pred_r = w.vel_r;  // the prediction by slot (see k_finalize_predict)
   FX_TRY(exclusive_scan_i32(w.cnt, w.off, nc, w.partial, st));
   if (last_in_call && s.cell_start && s.cell_end) LAUNCH(k_cell_ranges, nc, st, nc, w.off, s.cell_start, s.cell_end);
-  LAUNCH(k_scatter_pairs, n, st, n, (const int32_t*)w.key, (const int32_t*)w.rank, (const int32_t*)w.off,
-         (const int32_t*)w.sidx, w.pair64);
+  // also zeroes the counts (nc + 1) and flags[0], and invalidates the lists
+  LAUNCH(k_scatter_pairs, std::max(n, nc + 1), st, n, (const int32_t*)w.key, (const int32_t*)w.rank,
+         (const int32_t*)w.off, (const int32_t*)w.sidx, w.pair64, w.cnt, nc + 1, w.flags);
   // big-cell queue in the rank array (dead after the scatter)
   FX_TRY(canon_pairs(w.off, w.pair64, nc, w.rank, w.flags, st));
   LAUNCH(k_gather_pairs, n, st, n, (const uint64_t*)w.pair64, (const int32_t*)w.key, pred_r,
          (const float4*)w.pos_r, w.pred_s[target], w.pos_s, w.dl_s, last_in_call ? s.sorted_key : nullptr, w.skey,
          w.sidx, last_in_call ? s.sorted_idx : nullptr, last_in_call ? s.key : nullptr);
   forget_normals(s.work);
-  FX_TRY(fill(w.flags + 2, 1, kInfBits, st));
   FX_TRY(check_launch("build_grid_resident"));
   mark_grid(s.work, n, nc);
   return FX_OK;
@@ -1960,8 +1970,9 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
               L, Scenes{nullptr, 0});
     }
     if (resident_out) {
-      // finalize, and the next substep's predict (see k_finalize_predict)
-      FX_TRY(fill(w.cnt, p.n_cells + 1, 0, st));
+      // finalize, and the next substep's predict (see k_finalize_predict);
+      // a resident grid build has already zeroed the counts
+      if (!resident_in) FX_TRY(fill(w.cnt, p.n_cells + 1, 0, st));
       const bool cohesion = p.surface_tension > 0.0f;
       LAUNCH(k_finalize_predict, n, st, n, p, col, g, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
              (const float4*)w.pred_s[1], cohesion ? (const float4*)w.normal_next : (const float4*)nullptr,
