@@ -103,24 +103,38 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const int32_t* __re
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partial[nb];
 }
 
-// Up to kSmallScanMax items, one block scans everything in one launch (each
-// thread kSmallScanItems consecutive items): a small scene's cell offsets
-// then cost one launch instead of three.
+// Up to kSmallScanMax items, one block scans everything in one launch: a
+// small scene's cell offsets then cost one launch instead of three.  The
+// counts pass through LDS so that global loads and stores stay coalesced
+// (item i * 1024 + thread); each thread then scans per = ceil(n / 1024)
+// consecutive items in LDS.  One pad word per 32 items spreads those
+// stride-per reads over the banks.  Up to 1024 items (one per thread) the
+// direct loads are coalesced already and skip the staging.
 constexpr int kSmallScanItems = 16;
 constexpr long long kSmallScanMax = 1024LL * kSmallScanItems;
 
+__device__ __forceinline__ int scan_slot(int j) { return j + (j >> 5); }
+
 __global__ __launch_bounds__(1024) void k_scan_small(const int32_t* __restrict__ in, int32_t* __restrict__ out,
                                                      int n) {
+  __shared__ int data[kSmallScanMax + kSmallScanMax / 32];
   __shared__ int lds[16];
   const int per = (n + 1023) / 1024;  // <= kSmallScanItems
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int begin = threadIdx.x * per;
-  int v[kSmallScanItems];
   int s = 0;
+  if (per == 1) {
+    s = begin < n ? in[begin] : 0;  // one item per thread: already coalesced
+  } else {
 #pragma unroll
-  for (int i = 0; i < kSmallScanItems; ++i) {
-    v[i] = (i < per && begin + i < n) ? in[begin + i] : 0;
-    s += v[i];
+    for (int i = 0; i < kSmallScanItems; ++i) {
+      const int j = i * 1024 + (int)threadIdx.x;
+      if (j < n) data[scan_slot(j)] = in[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSmallScanItems; ++i)
+      if (i < per && begin + i < n) s += data[scan_slot(begin + i)];
   }
   int incl = s;
 #pragma unroll
@@ -132,10 +146,23 @@ __global__ __launch_bounds__(1024) void k_scan_small(const int32_t* __restrict__
   __syncthreads();
   for (int w = 0; w < wave; ++w) incl += lds[w];
   int run = incl - s;
+  if (per == 1) {
+    if (begin < n) out[begin] = run;
+  } else {
 #pragma unroll
-  for (int i = 0; i < kSmallScanItems; ++i) {
-    if (i < per && begin + i < n) out[begin + i] = run;
-    run += v[i];
+    for (int i = 0; i < kSmallScanItems; ++i) {
+      if (i < per && begin + i < n) {
+        const int v = data[scan_slot(begin + i)];
+        data[scan_slot(begin + i)] = run;
+        run += v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSmallScanItems; ++i) {
+      const int j = i * 1024 + (int)threadIdx.x;
+      if (j < n) out[j] = data[scan_slot(j)];
+    }
   }
   if (threadIdx.x == 1023) out[n] = incl;
 }
