@@ -153,9 +153,11 @@ class Solver:
 
     def capture(self, dt, substeps):
         """Capture `substeps` substeps of length dt in a HIP graph and return
-        it; graph.replay() then runs them with one host call.  One warm-up
-        substep runs eagerly first (capture itself executes nothing), so the
-        state advances by one substep here."""
+        it; graph.replay() then runs them with one host call.  They are
+        recorded as one fx_step call, so consecutive substeps hand their state
+        on by slot as an eager substeps_n call does.  One warm-up substep runs
+        eagerly first (capture itself executes nothing), so the state advances
+        by one substep here."""
         self._require_seeded()
         self.last_dt = dt
         torch.cuda.synchronize(self.device)
@@ -167,8 +169,7 @@ class Solver:
         torch.cuda.synchronize(self.device)
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
-            for _ in range(substeps):
-                self.substep(dt)
+            self.substeps_n(dt, substeps)
         return graph
 
     def step(self, frame_dt):
