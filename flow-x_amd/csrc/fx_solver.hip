@@ -428,9 +428,13 @@ __global__ void k_cell_ranges(int nc, const int32_t* __restrict__ off, int32_t* 
   end[c] = e > s ? e : -1;
 }
 
+// Counting-sort scatter.  The counts are dead once scanned: the same launch
+// (over max(n, n_cnt) threads) zeroes them for the next counting pass.
 __global__ void k_scatter(int n, const int32_t* __restrict__ key, const int32_t* __restrict__ rank,
-                          const int32_t* __restrict__ off, int32_t* __restrict__ sidx) {
+                          const int32_t* __restrict__ off, int32_t* __restrict__ sidx, int32_t* __restrict__ cnt,
+                          int n_cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_cnt) cnt[i] = 0;
   if (i >= n) return;
   const int k = key[i];
   sidx[off[k] + rank[i]] = i;
@@ -1797,14 +1801,14 @@ static int canon_pairs(const int32_t* off, uint64_t* pairs, int n_cells, int32_t
 }
 
 // Counting sort of ws.key (counts already in ws.cnt, ranks in ws.rank) into
-// ws.sidx: scan, scatter, canonicalise.  Writes reference-format ranges if
+// ws.sidx: scan, scatter (which zeroes the counts again), canonicalise.  Writes reference-format ranges if
 // requested.
 static int sort_from_counts(const FxParams& p, const FxState& s, const Ws& w, hipStream_t st) {
   const int n = p.n, nc = p.n_cells;
   FX_TRY(exclusive_scan_i32(w.cnt, w.off, nc, w.partial, st));
   if (s.cell_start && s.cell_end) LAUNCH(k_cell_ranges, nc, st, nc, w.off, s.cell_start, s.cell_end);
   if (n > 0) {
-    LAUNCH(k_scatter, n, st, n, w.key, w.rank, w.off, w.sidx);
+    LAUNCH(k_scatter, std::max(n, nc + 1), st, n, w.key, w.rank, w.off, w.sidx, w.cnt, nc + 1);
     // big-cell queue reuses the rank array (dead after the scatter)
     FX_TRY(canon_cells(w.off, w.sidx, nc, w.rank, w.flags, st));
   }
@@ -1971,8 +1975,7 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
     }
     if (resident_out) {
       // finalize, and the next substep's predict (see k_finalize_predict);
-      // a resident grid build has already zeroed the counts
-      if (!resident_in) FX_TRY(fill(w.cnt, p.n_cells + 1, 0, st));
+      // either grid build's scatter has already zeroed the counts
       const bool cohesion = p.surface_tension > 0.0f;
       LAUNCH(k_finalize_predict, n, st, n, p, col, g, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
              (const float4*)w.pred_s[1], cohesion ? (const float4*)w.normal_next : (const float4*)nullptr,
