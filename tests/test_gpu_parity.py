@@ -89,6 +89,27 @@ def test_build_grid_matches_oracle(fx, oracle, scene):
     assert np.array_equal(g.get("cell_end"), end)
 
 
+@pytest.mark.parametrize("dims", [(8, 8, 16), (5, 5, 41), (16, 32, 32), (5, 29, 113)])
+def test_build_grid_at_scan_size_boundaries(fx, oracle, scene, dims):
+    """Cell counts of 1024 / 1025 and 16384 / 16385 straddle the one-block
+    scan's direct form (one count per thread), its LDS-staged form and the
+    multi-block scan.  The dam-break state binned into each grid
+    (coordinates clamp to the dims, so the small grids hold runs of
+    thousands for the block-wide sort) must give the oracle's order and
+    ranges."""
+    p0, st, n = scene["p"], scene["st"], scene["n"]
+    p = type(p0).from_buffer_copy(p0)
+    p.cells[:] = list(dims)
+    p.n_cells = dims[0] * dims[1] * dims[2]
+    g = GpuState(fx, p, pred=st.pred, pos=st.pos)
+    g.run("build_grid")
+    keys, idx, start, end = _stable_grid(oracle, p, st.pred, n)
+    assert np.array_equal(g.get("sorted_key"), keys)
+    assert np.array_equal(g.get("sorted_idx"), idx)
+    assert np.array_equal(g.get("cell_start"), start)
+    assert np.array_equal(g.get("cell_end"), end)
+
+
 @pytest.mark.parametrize("sigma", [0.0, 0.0728])
 def test_predict_bit_exact(fx, oracle, scene, sigma):
     p, st = scene["p"], scene["st"]
