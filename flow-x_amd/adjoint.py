@@ -50,6 +50,9 @@ class DiffSolver:
             raise RuntimeError("fx_adjoint_workspace_bytes rejected the parameters")
         self.work = torch.zeros(int(need), dtype=torch.uint8, device=self.device)
         self.grad_params = torch.zeros(4, dtype=torch.float64, device=self.device)
+        # set on the device by any backward substep whose adjoints are not all
+        # finite; read once, by the backward of a rollout's first substep
+        self.nonfinite = torch.zeros((), dtype=torch.bool, device=self.device)
         self.surface = resolve(cfg, surface_resolution or cfg.extra.get("surface_resolution", 64), 0.5,
                                max_samples=1 << 30)
         self.sample_mask = self._sample_mask(np.zeros(3))
@@ -134,17 +137,33 @@ class DiffSolver:
         return gx
 
 
+class NonFiniteError(RuntimeError):
+    """A differentiable rollout produced a non-finite state, loss or gradient."""
+
+
+def _all_finite(*ts):
+    """Device-side: whether every element of every tensor is finite (no sync)."""
+    ok = torch.ones((), dtype=torch.bool, device=ts[0].device)
+    for t in ts:
+        ok = ok & torch.isfinite(t).all()
+    return ok
+
+
 class Substep(torch.autograd.Function):
     """(x, v, gravity, translation) -> (x', v'); x, v: N x 4 tensors.
-    decisions/mode: see DiffSolver.forward_raw; backward replays them."""
+    decisions/mode: see DiffSolver.forward_raw; backward replays them.
+    first: this is the first substep of a rollout, so its backward is the
+    last of the chain - it raises NonFiniteError if any backward substep of
+    the chain produced a non-finite adjoint (flagged on the device as they
+    run, read back once here)."""
 
     @staticmethod
-    def forward(ctx, x, v, gravity, translation, solver, decisions=None, mode=0):
+    def forward(ctx, x, v, gravity, translation, solver, decisions=None, mode=0, first=False):
         g = float(gravity)
         tr = [float(t) for t in translation]
         x_out, v_out = solver.forward_raw(x.contiguous(), v.contiguous(), g, tr, decisions, mode)
         ctx.save_for_backward(x, v)  # the substep's checkpoint
-        ctx.solver, ctx.g, ctx.tr, ctx.dec = solver, g, tr, decisions
+        ctx.solver, ctx.g, ctx.tr, ctx.dec, ctx.first = solver, g, tr, decisions, first
         ctx.gdtype = gravity.dtype, translation.dtype
         return x_out, v_out
 
@@ -155,8 +174,16 @@ class Substep(torch.autograd.Function):
             gx_out = torch.zeros_like(x)
         if gv_out is None:
             gv_out = torch.zeros_like(v)
-        gx, gv, gp = ctx.solver.backward_raw(x, v, ctx.g, ctx.tr, gx_out, gv_out, ctx.dec)
-        return gx, gv, gp[0].to(ctx.gdtype[0]), gp[1:4].to(ctx.gdtype[1]), None, None, None
+        solver = ctx.solver
+        gx, gv, gp = solver.backward_raw(x, v, ctx.g, ctx.tr, gx_out, gv_out, ctx.dec)
+        solver.nonfinite |= ~_all_finite(gx, gv, gp)
+        if ctx.first:
+            bad = bool(solver.nonfinite.item())
+            solver.nonfinite.zero_()
+            if bad:
+                raise NonFiniteError("the adjoint of the rollout is not finite (the substep map amplified the "
+                                     "output adjoint past the floating-point range): shorten the horizon")
+        return gx, gv, gp[0].to(ctx.gdtype[0]), gp[1:4].to(ctx.gdtype[1]), None, None, None, None
 
 
 class Splat(torch.autograd.Function):
@@ -179,8 +206,11 @@ def rollout(solver, x0, v0, gravity, translation, substeps, decisions=None, mode
     with solver.new_decisions(), keep allocation out of the loop), else a new
     record is appended.  Mode 2 replays a list recorded earlier.  Raises if a
     neighbour list overflowed its capacity, since the gradient of a truncated
-    list is not that of the substep."""
+    list is not that of the substep, and raises NonFiniteError if the final
+    state is not finite.  The backward of the rollout raises NonFiniteError
+    when any substep's adjoint is not finite."""
     x, v = x0, v0
+    solver.nonfinite.zero_()
     for k in range(substeps):
         dec = None
         if decisions is not None:
@@ -192,9 +222,11 @@ def rollout(solver, x0, v0, gravity, translation, substeps, decisions=None, mode
                     decisions.append(dec)
             elif mode == 2:
                 dec = decisions[k]
-        x, v = Substep.apply(x, v, gravity, translation, solver, dec, mode)
+        x, v = Substep.apply(x, v, gravity, translation, solver, dec, mode, k == 0)
     if check_overflow and solver.overflowed():
         raise RuntimeError("adjoint neighbour lists overflowed their capacity (local density too high)")
+    if substeps > 0 and not bool(_all_finite(x, v).item()):
+        raise NonFiniteError(f"the rollout's state is not finite after {substeps} substeps")
     return x, v
 
 
@@ -214,4 +246,7 @@ def loss_fn(solver, x, target, com_target, com_weight=1.0):
     centre-of-mass term."""
     field = Splat.apply(x, solver)
     com = x[:, :3].mean(0)
-    return ((field - target) ** 2).mean() + com_weight * ((com - com_target) ** 2).sum()
+    loss = ((field - target) ** 2).mean() + com_weight * ((com - com_target) ** 2).sum()
+    if not bool(torch.isfinite(loss).item()):
+        raise NonFiniteError("the loss is not finite")
+    return loss

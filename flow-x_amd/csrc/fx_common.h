@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <string>
 
 #include "flowx.h"
@@ -288,12 +289,63 @@ struct Tuning {
 };
 Tuning tuning();
 
-// Host-side record of which workspaces hold a built grid, and for which
-// (n, n_cells): passes that read the previous grid (the normal pass, the
-// splat) refuse a workspace that never had one instead of reading garbage
-// offsets.
-void mark_grid(const void* work, int n, int n_cells);
-bool grid_ready(const void* work, int n, int n_cells);
+// Workspace header: words kHdr.. of the workspace's flags block record what
+// the workspace holds.  The kernels that build a grid or compute normals write
+// it, so the record travels with the memory, not with its address: a fresh or
+// reused allocation carries no claim it cannot back.
+//   magic, n, n_cells   a grid (offsets, sorted slots, sorted-space state)
+//                       for n particles over n_cells cells is built
+//   normals             0: none; 1: the next substep's normals are in the
+//                       workspace's normal_next (by particle); 2: they are
+//                       already in the caller's normal buffer
+//   h, mass, nlo, nhi   the smoothing radius and mass (float bits) and the
+//                       normal buffer's address those normals were made for
+//   error               set by a kernel that found no grid (a captured launch
+//                       on a workspace that never had one); cleared by the
+//                       next grid build
+// Passes that read the previous grid (the normal pass, the splat) check it on
+// the device, and the host entry points read it back when they are not under
+// stream capture.
+constexpr int32_t kGridMagic = 0x31475846;  // "FXG1"
+enum HeaderWord : int {
+  kHdr = 16,  // first word, in the flags block
+  kHdrMagic = kHdr,
+  kHdrN,
+  kHdrCells,
+  kHdrNormals,
+  kHdrH,
+  kHdrMass,
+  kHdrNormLo,
+  kHdrNormHi,
+  kHdrError,
+  kHdrEnd
+};
+
+__device__ __forceinline__ bool header_grid_ok(const int32_t* flags, int n, int nc) {
+  return flags[kHdrMagic] == kGridMagic && flags[kHdrN] == n && flags[kHdrCells] == nc;
+}
+
+// What a usable normals record must carry (normals computed for this
+// normal buffer with these constants).
+struct NormalKey {
+  int32_t hbits, mbits;
+  uint32_t lo, hi;
+};
+inline NormalKey normal_key(const FxParams& p, const void* normal) {
+  NormalKey k;
+  memcpy(&k.hbits, &p.h, 4);
+  memcpy(&k.mbits, &p.mass, 4);
+  const uint64_t a = (uint64_t)(uintptr_t)normal;
+  k.lo = (uint32_t)a;
+  k.hi = (uint32_t)(a >> 32);
+  return k;
+}
+
+// Host check that the workspace at `work` (solver layout wl) holds a grid
+// for (n, n_cells): a synchronous read of the header, skipped while `st` is
+// capturing (the kernels check it themselves then).  `what` names the caller
+// in the error.
+int require_grid(const Workspace& wl, const void* work, int n, int n_cells, hipStream_t st, const char* what);
 
 // Exclusive scan of int32 counts[0..n) into out[0..n] (out[n] = total).
 // partial must hold scan_partials(n) ints.  Launches 3 kernels.

@@ -87,55 +87,24 @@ static Tuning g_tuning = {0.0f, 64};
 
 Tuning tuning() { return g_tuning; }
 
-static std::mutex g_grid_mu;
-static std::unordered_map<const void*, std::pair<int, int>> g_grids;
-
-void mark_grid(const void* work, int n, int n_cells) {
-  std::lock_guard<std::mutex> lock(g_grid_mu);
-  g_grids[work] = std::make_pair(n, n_cells);
-}
-
-bool grid_ready(const void* work, int n, int n_cells) {
-  std::lock_guard<std::mutex> lock(g_grid_mu);
-  auto it = g_grids.find(work);
-  return it != g_grids.end() && it->second.first == n && it->second.second == n_cells;
-}
-
-// Which workspaces already hold the normals the next substep's normal pass
-// would compute: the XSPH pass of a substep traverses the same grid and the
-// same final predictions with the same densities as the next substep's
-// normal pass (shaders/sph_normal.glsl:15-21 reads the previous substep's
-// state), so with cohesion on it computes both, into the workspace's
-// normal_next; the next substep copies them into the caller's normal buffer
-// where its normal pass would have written them.  Between two substeps of one
-// fx_step call nobody reads the normal buffer, so there the XSPH pass writes
-// straight into it (in_place) and the copy goes.  Any other grid build or
-// state load forgets the entry.  Keyed by the normal buffer and the
-// constants the pass uses.
-struct NormalKey {
-  const void* normal;
-  float h, mass;
-  bool in_place;
-};
-static std::unordered_map<const void*, NormalKey> g_normals;
-
-static void mark_normals(const void* work, const void* normal, const FxParams& p, bool in_place) {
-  std::lock_guard<std::mutex> lock(g_grid_mu);
-  g_normals[work] = NormalKey{normal, p.h, p.mass, in_place};
-}
-
-static void forget_normals(const void* work) {
-  std::lock_guard<std::mutex> lock(g_grid_mu);
-  g_normals.erase(work);
-}
-
-// 0: not computed, 1: in the workspace's normal_next, 2: already in `normal`
-static int normals_ready(const void* work, const void* normal, const FxParams& p) {
-  std::lock_guard<std::mutex> lock(g_grid_mu);
-  auto it = g_normals.find(work);
-  if (it == g_normals.end() || it->second.normal != normal || it->second.h != p.h || it->second.mass != p.mass)
-    return 0;
-  return it->second.in_place ? 2 : 1;
+int require_grid(const Workspace& wl, const void* work, int n, int n_cells, hipStream_t st, const char* what) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  FX_HIP_CHECK(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone) return FX_OK;
+  int32_t h[kHdrEnd - kHdr];
+  const char* src = (const char*)work + wl.flags + kHdr * sizeof(int32_t);
+  FX_HIP_CHECK(hipMemcpyAsync(h, src, sizeof(h), hipMemcpyDeviceToHost, st));
+  FX_HIP_CHECK(hipStreamSynchronize(st));
+  // (the error word is set only by a kernel that found no grid, and every
+  // grid build clears it, so it matters only when no grid is recorded)
+  if (h[kHdrMagic - kHdr] != kGridMagic || h[kHdrN - kHdr] != n || h[kHdrCells - kHdr] != n_cells)
+    return fail(FX_EINVAL,
+                "%s reads the previous grid, and this workspace holds none for %d particles / %d cells: "
+                "call fx_load_state (or fx_build_grid) first%s",
+                what, n, n_cells,
+                h[kHdrError - kHdr] == 1 ? " (a captured launch already ran on it: its normals or field are zero)"
+                                         : "");
+  return FX_OK;
 }
 
 // ---- constants (shaders/sph_common.glsl:68-91, sph_delta.glsl:44) --------
@@ -499,9 +468,29 @@ __global__ void k_scatter_pairs(int n, const int32_t* __restrict__ key, const in
   if (t == 0) {
     flags[0] = 0;
     flags[2] = kInfBits;
+    // the workspace header: this grid, and no normals by particle (the
+    // resident substeps keep theirs by slot)
+    flags[kHdrMagic] = kGridMagic;
+    flags[kHdrN] = n;
+    flags[kHdrCells] = n_cnt - 1;
+    flags[kHdrNormals] = 0;
+    flags[kHdrError] = 0;
   }
   if (t >= n) return;
   pairs[off[key[t]] + rank[t]] = ((uint64_t)(uint32_t)pid[t] << 32) | (uint32_t)t;
+}
+
+// The closing mark of a grid build: the neighbour lists are invalid
+// (flags[2]) and the workspace header names the new grid, forgets any
+// normals computed over the previous one and clears the error word.
+__global__ void k_mark_grid(int32_t* __restrict__ flags, int n, int nc) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  flags[2] = kInfBits;
+  flags[kHdrMagic] = kGridMagic;
+  flags[kHdrN] = n;
+  flags[kHdrCells] = nc;
+  flags[kHdrNormals] = 0;
+  flags[kHdrError] = 0;
 }
 
 // The gather of a resident build: slot u of the new order reads its previous
@@ -1510,6 +1499,15 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
 // Keeping the collider search out of the neighbour pass keeps that pass's
 // register count, and so its occupancy, down (63 instead of 81 VGPRs).
 // V = 8: batched scenes (each particle's params and collider from its scene).
+// The record a substep leaves when its XSPH pass computed the next
+// substep's normals (see the workspace header in fx_common.h); flags NULL:
+// none.
+struct NormalMark {
+  int32_t* flags;
+  int32_t state;  // 1: in normal_next, 2: in the caller's normal buffer
+  NormalKey key;
+};
+
 template <int V>
 __global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p_, Collider col_,
                                                     const float4* __restrict__ pred_s,
@@ -1518,8 +1516,16 @@ __global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p_, Collider
                                                     const int32_t* __restrict__ sidx,
                                                     float4* __restrict__ delta4, float4* __restrict__ pos4,
                                                     float4* __restrict__ vel4, float4* __restrict__ pos_s,
-                                                    float4* __restrict__ pred4_out, Scenes S) {
+                                                    float4* __restrict__ pred4_out, Scenes S,
+                                                    NormalMark mk = NormalMark{}) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mk.flags && t == 0) {
+    mk.flags[kHdrNormals] = mk.state;
+    mk.flags[kHdrH] = mk.key.hbits;
+    mk.flags[kHdrMass] = mk.key.mbits;
+    mk.flags[kHdrNormLo] = (int32_t)mk.key.lo;
+    mk.flags[kHdrNormHi] = (int32_t)mk.key.hi;
+  }
   if (t >= n) return;
   FxParams p = p_;
   Collider col = col_;
@@ -1541,6 +1547,20 @@ __global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p_, Collider
   if (pos_s) pos_s[t] = pf;
 }
 
+// How a substep's normal pass decides on the device what to do (flags NULL:
+// always compute).  With no grid for (n, nc) in the workspace header it
+// writes zero normals and sets the header's error word instead of reading
+// offsets the workspace does not hold; when the header records normals made
+// for `key` it takes them (state 1: copies normal_next, state 2: they are in
+// place already) instead of traversing the stencil again.  The choice is the
+// kernel's, so a captured substep makes it afresh on every replay.
+struct NormalPick {
+  int32_t* flags;
+  const float4* next;
+  NormalKey key;
+  int nc;
+};
+
 // shaders/sph_normal.glsl:24-81 over the previous grid; pred_s.w = the
 // neighbour's density from the last lambda pass.  Writes canonical normal4.
 template <int V>
@@ -1548,9 +1568,23 @@ __global__ __launch_bounds__(256) void k_normal(int n, Grid g_, Consts k_,
                                                 const float4* __restrict__ pred_s,
                                                 const int32_t* __restrict__ off_,
                                                 const int32_t* __restrict__ sidx,
-                                                float4* __restrict__ normal4, Lists L, Scenes S) {
+                                                float4* __restrict__ normal4, Lists L, Scenes S,
+                                                NormalPick pk = NormalPick{}) {
   constexpr int MODE = V & 3;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pk.flags) {
+    if (!header_grid_ok(pk.flags, n, pk.nc)) {
+      if (t < n) normal4[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (t == 0) pk.flags[kHdrError] = 1;
+      return;
+    }
+    const int state = pk.flags[kHdrNormals];
+    if (state != 0 && pk.flags[kHdrH] == pk.key.hbits && pk.flags[kHdrMass] == pk.key.mbits &&
+        (uint32_t)pk.flags[kHdrNormLo] == pk.key.lo && (uint32_t)pk.flags[kHdrNormHi] == pk.key.hi) {
+      if (state == 1 && t < n) normal4[t] = pk.next[t];
+      return;
+    }
+  }
   if (t >= n) return;
   Grid g = g_;
   Consts k = k_;
@@ -1836,11 +1870,8 @@ static int build_grid(const FxParams& p, FxState& s, const Ws& w, bool do_predic
   }
   // a new grid invalidates the neighbour lists until a lambda pass rebuilds
   // them, and any normals computed for the next substep
-  forget_normals(s.work);
-  FX_TRY(fill(w.flags + 2, 1, kInfBits, st));
-  FX_TRY(check_launch("build_grid"));
-  mark_grid(s.work, n, nc);
-  return FX_OK;
+  FX_LAUNCH(k_mark_grid, dim3(1), dim3(64), st, w.flags, n, nc);
+  return check_launch("build_grid");
 }
 
 // Grid build of a substep whose state enters by slot of the previous
@@ -1866,10 +1897,7 @@ static int build_grid_resident(const FxParams& p, FxState& s, const Ws& w, int t
   LAUNCH(k_gather_pairs, n, st, n, (const uint64_t*)w.pair64, (const int32_t*)w.key, pred_r,
          (const float4*)w.pos_r, w.pred_s[target], w.pos_s, w.dl_s, last_in_call ? s.sorted_key : nullptr, w.skey,
          w.sidx, last_in_call ? s.sorted_idx : nullptr, last_in_call ? s.key : nullptr);
-  forget_normals(s.work);
-  FX_TRY(check_launch("build_grid_resident"));
-  mark_grid(s.work, n, nc);
-  return FX_OK;
+  return check_launch("build_grid_resident");
 }
 
 // last: this substep's state is what the caller sees next.  Earlier
@@ -1898,21 +1926,19 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
   // sph_normal runs first, over the previous substep's grid and state
   // (shaders/sph_normal.glsl:15-21); its output feeds sph_predict.  A
   // resident substep takes the normals by slot from the previous XSPH pass.
-  if (p.surface_tension > 0.0f && !resident_in) {
+  // Whether the previous substep's XSPH pass already computed these normals
+  // (then they are copied or already in place) or the pass must run is read
+  // from the workspace header by the kernel itself (NormalPick).
+  if (p.surface_tension > 0.0f && !resident_in && n > 0) {
     if (!s.normal) return fail(FX_EINVAL, "surface_tension > 0 needs a normal buffer");
-    if (!grid_ready(s.work, n, p.n_cells))
-      return fail(FX_EINVAL, "surface tension reads the previous grid: call fx_load_state first");
+    const NormalPick pk{w.flags, w.normal_next, normal_key(p, s.normal), p.n_cells};
     // without lists the one-mode kernels need fewer registers (higher occupancy)
-    const int ready = n > 0 ? normals_ready(s.work, s.normal, p) : 0;
-    if (ready == 1) {
-      // computed by the previous substep's XSPH pass
-      FX_HIP_CHECK(hipMemcpyAsync(s.normal, w.normal_next, (size_t)n * 16, hipMemcpyDeviceToDevice, st));
-    } else if (ready == 2) {
-      // written in place by the previous substep's XSPH pass
-    } else if (n > 0 && lists)
-      NLAUNCH(k_normal<VISIT_LIST>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L, Scenes{nullptr, 0});
-    else if (n > 0)
-      NLAUNCH(k_normal<VISIT_FULL>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L, Scenes{nullptr, 0});
+    if (lists)
+      NLAUNCH(k_normal<VISIT_LIST>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L,
+              Scenes{nullptr, 0}, pk);
+    else
+      NLAUNCH(k_normal<VISIT_FULL>, n, st, n, g, k, w.pred_s[0], w.off, w.sidx, (float4*)s.normal, L,
+              Scenes{nullptr, 0}, pk);
   }
   const int iters = p.iterations;
   int cur = iters & 1;  // the last iteration writes buffer 0
@@ -1981,11 +2007,14 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
              (const float4*)w.pred_s[1], cohesion ? (const float4*)w.normal_next : (const float4*)nullptr,
              (const int32_t*)w.sidx, w.pos_r, w.vel_r, w.key, w.cnt, w.rank,
              (cohesion && next_is_last) ? (float4*)s.normal : (float4*)nullptr);
-    } else
+    } else {
+      // with fused normals, the header records where they went
+      const NormalMark mk = fuse_normals ? NormalMark{w.flags, last_in_call ? 1 : 2, normal_key(p, s.normal)}
+                                         : NormalMark{};
       LAUNCH(k_finalize_s<0>, n, st, n, p, col, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
              (const float4*)w.pred_s[1], w.sidx, delta_out, (float4*)s.pos, (float4*)s.vel, pos_s_out, pred_out,
-             Scenes{nullptr, 0});
-    if (fuse_normals && !resident_out) mark_normals(s.work, s.normal, p, !last_in_call);
+             Scenes{nullptr, 0}, mk);
+    }
   }
   return check_launch("substep");
 }
@@ -2000,11 +2029,8 @@ static int canonical_grid(const FxParams& p, const FxState& s, const Ws& w, hipS
     FX_HIP_CHECK(hipMemcpyAsync(w.skey, s.sorted_key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
   }
   FX_LAUNCH(k_off_from_sorted, dim3(grid_for((long long)n + 1)), dim3(kBlock), st, n, nc, w.skey, w.off);
-  forget_normals(s.work);
-  FX_TRY(fill(w.flags + 2, 1, kInfBits, st));
-  FX_TRY(check_launch("canonical_grid"));
-  mark_grid(s.work, n, nc);
-  return FX_OK;
+  FX_LAUNCH(k_mark_grid, dim3(1), dim3(64), st, w.flags, n, nc);
+  return check_launch("canonical_grid");
 }
 
 static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStream_t st) {
@@ -2049,7 +2075,7 @@ static int run_pass(int id, const FxParams& p, FxState& s, const Ws& w, hipStrea
       FxState t = s;
       t.cell_start = t.cell_end = nullptr;
       FX_TRY(sort_from_counts(p, t, w, st));
-      mark_grid(s.work, n, nc);
+      FX_LAUNCH(k_mark_grid, dim3(1), dim3(64), st, w.flags, n, nc);
       if (n > 0) {
         FX_HIP_CHECK(hipMemcpyAsync(s.sorted_idx, w.sidx, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
         LAUNCH(k_sorted_keys, n, st, n, w.sidx, w.key, w.skey);
@@ -2239,8 +2265,10 @@ static int batch_step(const FxScene* scenes, int count, float* pos, float* vel, 
   }
   const Lists L0 = make_lists(w, make_consts(scenes[0].params), false);
   if (!load && b.tension) {
-    if (!grid_ready(work, n, nc)) return fail(FX_EINVAL, "batch: surface tension needs fx_batch_load_state first");
-    NLAUNCH(k_normal<8 + VISIT_FULL>, n, st, n, Grid{}, Consts{}, w.pred_s[0], w.off, w.sidx, normal, L0, S);
+    FX_TRY(require_grid(workspace_layout(b.virt), wsp, n, nc, st, "batch: surface tension"));
+    // the pick only guards the grid: the batch never fuses normals into XSPH
+    const NormalPick pk{w.flags, nullptr, NormalKey{}, nc};
+    NLAUNCH(k_normal<8 + VISIT_FULL>, n, st, n, Grid{}, Consts{}, w.pred_s[0], w.off, w.sidx, normal, L0, S, pk);
   }
   const int iters = b.iterations;
   const int cur0 = load ? 0 : (iters & 1);
@@ -2253,7 +2281,7 @@ static int batch_step(const FxScene* scenes, int count, float* pos, float* vel, 
   if (n > 0)
     LAUNCH(k_gather, n, st, n, w.sidx, w.key, (const float4*)pred, (const float4*)pos, (const float4*)nullptr,
            w.pred_s[cur0], w.pos_s, w.dl_s, (int32_t*)nullptr, w.skey, (int32_t*)nullptr);
-  mark_grid(work, n, nc);
+  FX_LAUNCH(k_mark_grid, dim3(1), dim3(64), st, w.flags, n, nc);
   if (load || n == 0) return check_launch("batch");
   int cur = cur0;
   for (int it = 0; it < iters; ++it) {
@@ -2354,10 +2382,18 @@ int fx_build_grid(const FxParams* p, FxState* s, void* stream) {
   return build_grid(*p, *s, w, false, 0, (hipStream_t)stream);
 }
 
+// The normal pass of a call's first substep reads the previous grid
+// (shaders/sph_normal.glsl:15-21): refuse a workspace that holds none.
+static int check_previous_grid(const FxParams& p, const FxState& s, hipStream_t st, const char* what) {
+  if (!(p.surface_tension > 0.0f) || p.n == 0) return FX_OK;
+  return require_grid(workspace_layout(p), s.work, p.n, p.n_cells, st, what);
+}
+
 int fx_substep(const FxParams* p, FxState* s, void* stream) {
   FX_TRY(validate(p, s, true));
   if (p->n > 0 && (!s->pos || !s->vel)) return fail(FX_EINVAL, "substep needs pos and vel");
   if (p->n > 0 && !s->pred) return fail(FX_EINVAL, "substep needs a pred buffer");
+  FX_TRY(check_previous_grid(*p, *s, (hipStream_t)stream, "fx_substep with surface tension"));
   Ws w = bind_ws(workspace_layout(*p), s->work);
   return substep(*p, *s, w, (hipStream_t)stream);
 }
@@ -2368,6 +2404,7 @@ int fx_step(const FxParams* p, FxState* s, int substeps, void* stream) {
   FX_TRY(validate(p, s, true));
   if (p->n > 0 && (!s->pos || !s->vel)) return fail(FX_EINVAL, "substep needs pos and vel");
   if (p->n > 0 && !s->pred) return fail(FX_EINVAL, "substep needs a pred buffer");
+  FX_TRY(check_previous_grid(*p, *s, (hipStream_t)stream, "fx_step with surface tension"));
   Ws w = bind_ws(workspace_layout(*p), s->work);
   const bool resident = resident_ok(*p, *s);
   for (int i = 0; i < substeps; ++i) {

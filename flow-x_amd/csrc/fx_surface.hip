@@ -191,7 +191,8 @@ __global__ __launch_bounds__(256) void k_splat_m(FxSurface sf, Grid g, float mas
                                                  const float4* __restrict__ pos_s,
                                                  const int32_t* __restrict__ off,
                                                  const int32_t* __restrict__ dcell_bits, float kr,
-                                                 float* __restrict__ field) {
+                                                 float* __restrict__ field, const int32_t* __restrict__ hdr,
+                                                 int n) {
   constexpr int M = MQ % kSplatQuad;
   constexpr bool QUAD = MQ >= kSplatQuad;  // four lanes per thread's samples
   const int nx = sf.dims[0], ny = sf.dims[1], nz = sf.dims[2];
@@ -214,7 +215,10 @@ __global__ __launch_bounds__(256) void k_splat_m(FxSurface sf, Grid g, float mas
   }
   const float pad = 1e-3f * kr;
   const int wlo = max(bxs[0] - 2, 0), whi = min(bxs[M - 1] + 2, g.cx - 1);
-  for (int dz = -2; dz <= 2; ++dz) {
+  // a workspace without this grid (the header check of fx_common.h) leaves
+  // the field zero rather than read offsets it does not hold
+  const int zend = header_grid_ok(hdr, n, g.nc) ? 2 : -3;
+  for (int dz = -2; dz <= zend; ++dz) {
     const int z = bz + dz;
     if (z < 0 || z >= g.cz) continue;
     const float zlo = g.loz + (float)z * g.cell;
@@ -318,8 +322,12 @@ __global__ __launch_bounds__(256) void k_splat_m(FxSurface sf, Grid g, float mas
 // cell's slot: contention only among one cell's particles).
 __global__ void k_gather_pos(int n, const int32_t* __restrict__ sidx, const int32_t* __restrict__ skey,
                              Grid g, const float4* __restrict__ pos, float4* __restrict__ pos_s,
-                             int32_t* __restrict__ dcell_bits) {
+                             int32_t* __restrict__ dcell_bits, int32_t* __restrict__ hdr) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!header_grid_ok(hdr, n, g.nc)) {
+    if (t == 0) hdr[kHdrError] = 1;
+    return;
+  }
   float d = 0.0f;
   if (t < n) {
     const float4 p = pos[sidx[t]];
@@ -782,12 +790,12 @@ int fx_splat(const FxParams* p, const FxSurface* surf, const FxState* s, float* 
     return fail(FX_EINVAL, "fx_splat: surface dims must be >= 1");
   if (!s->work || s->work_bytes < workspace_layout(*p).total)
     return fail(FX_ENOMEM, "fx_splat: workspace missing or too small");
-  if (!grid_ready(s->work, p->n, p->n_cells))
-    return fail(FX_EINVAL, "fx_splat gathers through the last grid: build one first");
   if (p->n > 0 && !s->pos) return fail(FX_EINVAL, "fx_splat: pos is NULL");
   hipStream_t st = (hipStream_t)stream;
   const Workspace wl = workspace_layout(*p);
+  FX_TRY(require_grid(wl, s->work, p->n, p->n_cells, st, "fx_splat (it gathers through the last grid)"));
   char* b = (char*)s->work;
+  int32_t* hdr = (int32_t*)(b + wl.flags);
   const int32_t* sidx = (const int32_t*)(b + wl.sidx);
   const int32_t* off = (const int32_t*)(b + wl.off);
   const int32_t* skey = (const int32_t*)(b + wl.skey);
@@ -796,7 +804,7 @@ int fx_splat(const FxParams* p, const FxSurface* surf, const FxState* s, float* 
   FX_LAUNCH(k_fill_cells, dim3(grid_for(p->n_cells)), dim3(kBlock), st, dmax, p->n_cells);
   if (p->n > 0)
     FX_LAUNCH(k_gather_pos, dim3(grid_for(p->n)), dim3(kBlock), st, p->n, sidx, skey, make_grid(*p),
-              (const float4*)s->pos, pos_s, dmax);
+              (const float4*)s->pos, pos_s, dmax, hdr);
   const long long S = (long long)surf->dims[0] * surf->dims[1] * surf->dims[2];
   const float kr = surf->kernel_radius;
   const float coef = poly6_coef(kr);
@@ -807,11 +815,11 @@ int fx_splat(const FxParams* p, const FxSurface* surf, const FxState* s, float* 
   if (threads < kSplatQuadBelow)
     FX_LAUNCH(k_splat_m<M + kSplatQuad>, dim3(grid_for(4 * threads)), dim3(256), st, *surf, make_grid(*p), p->mass,
               p->rest_density, kr * kr, coef, p->max_cell_scan > 0 ? p->max_cell_scan : 512, col,
-              (const float4*)pos_s, off, (const int32_t*)dmax, kr, field);
+              (const float4*)pos_s, off, (const int32_t*)dmax, kr, field, (const int32_t*)hdr, p->n);
   else
     FX_LAUNCH(k_splat_m<M>, dim3(grid_for(threads)), dim3(256), st, *surf, make_grid(*p), p->mass,
               p->rest_density, kr * kr, coef, p->max_cell_scan > 0 ? p->max_cell_scan : 512, col,
-              (const float4*)pos_s, off, (const int32_t*)dmax, kr, field);
+              (const float4*)pos_s, off, (const int32_t*)dmax, kr, field, (const int32_t*)hdr, p->n);
   (void)S;
   return check_launch("fx_splat");
 }

@@ -155,9 +155,12 @@ class Solver:
         """Capture `substeps` substeps of length dt in a HIP graph and return
         it; graph.replay() then runs them with one host call.  They are
         recorded as one fx_step call, so consecutive substeps hand their state
-        on by slot as an eager substeps_n call does.  One warm-up substep runs
-        eagerly first (capture itself executes nothing), so the state advances
-        by one substep here."""
+        on by slot as an eager substeps_n call does.  What the recorded
+        substeps find in the workspace (whether the normals of the previous
+        substep are already computed) is decided by the kernels on every
+        replay, so load_state / reset between replays are safe.  One warm-up
+        substep runs eagerly first (capture itself executes nothing), so the
+        state advances by one substep here."""
         self._require_seeded()
         self.last_dt = dt
         torch.cuda.synchronize(self.device)
@@ -189,20 +192,29 @@ class Solver:
             raise RuntimeError("Solver.reset() or load_state() must run before stepping")
 
     # ---- read-back (solver/gpu_util.py:62-89) ------------------------------
-    def read_positions(self):
+    # Every read-back checks that what it returns is finite (check=False
+    # skips it, e.g. to inspect a run that has diverged).
+    @staticmethod
+    def _finite(a, what, check):
+        if check and not np.all(np.isfinite(a)):
+            bad = int(np.count_nonzero(~np.all(np.isfinite(a.reshape(len(a), -1)), axis=1)))
+            raise RuntimeError(f"{what}: {bad} of {len(a)} particles are not finite (the run diverged)")
+        return a
+
+    def read_positions(self, check=True):
         """N x 4 float32 (w = 1), as read_texture(positions_img) returns."""
-        return self.pos.cpu().numpy()
+        return self._finite(self.pos.cpu().numpy(), "positions", check)
 
-    def read_velocities(self):
+    def read_velocities(self, check=True):
         """N x 4 float32 (w = 0)."""
-        return self.vel.cpu().numpy()
+        return self._finite(self.vel.cpu().numpy(), "velocities", check)
 
-    def read_density(self):
+    def read_density(self, check=True):
         """lambda_img.x: density from the last Jacobi lambda pass
         (scripts/smoke_test.py:117-134)."""
         if self.lam is None:
             raise RuntimeError("read_density needs keep_scratch=True")
-        return self.lam[:, 0].cpu().numpy()
+        return self._finite(self.lam[:, 0].cpu().numpy(), "densities", check)
 
     def stats(self):
         """solver/sph.py:862-886 stats(), minus the Blender timeline fields."""
