@@ -95,7 +95,13 @@ class ColliderGrid:
 
 def occupancy_cpu(origin, voxel, dims, shapes):
     """numpy reference of the voxel-centre test (used by host-side callers that
-    have no GPU, e.g. cache hashing of shape lists)."""
+    have no GPU, e.g. cache hashing of shape lists).  Inputs are rounded to
+    float32 as the C ABI receives them (origin, voxel size, shape
+    parameters); the voxel centres and the tests are float64, as on the GPU."""
+    origin = vec32(origin)
+    voxel = f32(voxel)
+    shapes = [(s[0],) + tuple(vec32(a) if isinstance(a, (tuple, list)) else f32(a) for a in s[1:])
+              for s in shapes]
     nx, ny, nz = dims
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     x = origin[0] + (i + 0.5) * voxel
