@@ -132,6 +132,10 @@ SIGNATURES = {
         [_VP, _VP, _VP, ctypes.c_double, ctypes.c_float, _VP, ctypes.c_int64, _VP, ctypes.c_int64, _VP, _VP,
          _VP, ctypes.c_size_t, _VP],
     ),
+    "fx_mc_extract_static": (
+        _I,
+        [_VP, _VP, _VP, ctypes.c_double, ctypes.c_float, _VP, ctypes.c_int64, _VP, ctypes.c_int64, _VP, _VP, _VP],
+    ),
     "fx_mc_tables": (_I, [_VP, _VP]),
     "fx_voxelize_boxes": (_I, [_VP, _VP, ctypes.c_float, _VP, _I, _VP, _I, _VP]),
     "fx_voxelize_sphere": (_I, [_VP, _VP, ctypes.c_float, _VP, ctypes.c_float, _VP, _I, _VP]),

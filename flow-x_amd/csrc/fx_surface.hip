@@ -12,6 +12,7 @@
 // reversed fans.  Vertex positions are computed in float64 without
 // contraction and rounded once to float32.
 #include <algorithm>
+#include <mutex>
 
 #ifndef FX_SPLAT_M
 #define FX_SPLAT_M 2  // lattice samples per splat thread (1, 2, 4, 8 timed: 2 fastest)
@@ -778,6 +779,78 @@ static DedupWs dedup_layout(long long n) {
   return w;
 }
 
+// Marching cubes of `field` with its scratch (mc_layout) at `b`: classify,
+// two scans, vertices, triangles, counts; asynchronous on st.
+static int mc_extract_run(const char* name, const float* field, const int32_t dims[3], const double origin[3],
+                          double spacing, float iso, float* verts, int64_t vcap, int32_t* tris, int64_t tcap,
+                          int64_t* counts, uint8_t* cases, char* b, size_t work_bytes, hipStream_t st) {
+  if (!field || !dims || !origin || !counts) return fail(FX_EINVAL, "%s: null argument", name);
+  if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return fail(FX_EINVAL, "%s: dims must be >= 1", name);
+  if (vcap < 0 || tcap < 0) return fail(FX_EINVAL, "%s: negative capacity", name);
+  if ((vcap > 0 && !verts) || (tcap > 0 && !tris)) return fail(FX_EINVAL, "%s: null output", name);
+  const long long S = (long long)dims[0] * dims[1] * dims[2];
+  if (S > (1LL << 28)) return fail(FX_ERANGE, "%s: %lld samples exceed 2^28", name, S);
+  const McWs wl = mc_layout(S);
+  if (work_bytes < wl.total) return fail(FX_ENOMEM, "%s: workspace %zu < %zu", name, work_bytes, wl.total);
+  uint16_t* code = (uint16_t*)(b + wl.code);
+  uint32_t* vbase = (uint32_t*)(b + wl.vbase);
+  uint32_t* tbase = (uint32_t*)(b + wl.tbase);
+  int32_t* bv = (int32_t*)(b + wl.bv);
+  int32_t* bt = (int32_t*)(b + wl.bt);
+  int32_t* bv_off = (int32_t*)(b + wl.bv_off);
+  int32_t* bt_off = (int32_t*)(b + wl.bt_off);
+  int32_t* part = (int32_t*)(b + wl.part);
+  const long long nb = (S + kMcBlock - 1) / kMcBlock;
+  McLattice L{dims[0], dims[1], dims[2], S, iso};
+  FX_LAUNCH(k_mc_classify, dim3((unsigned)nb), dim3(kMcBlock), st, L, field, code, bv, bt, cases);
+  FX_TRY(exclusive_scan_i32(bv, bv_off, nb, part, st));
+  FX_TRY(exclusive_scan_i32(bt, bt_off, nb, part, st));
+  FX_LAUNCH(k_mc_vertices, dim3((unsigned)nb), dim3(kMcBlock), st, L, field,
+                     (const uint16_t*)code, (const int32_t*)bv_off, (const int32_t*)bt_off, vbase, tbase,
+                     origin[0], origin[1], origin[2], spacing, verts, (long long)vcap);
+  FX_LAUNCH(k_mc_triangles, dim3((unsigned)nb), dim3(kMcBlock), st, L, (const uint16_t*)code,
+                     (const uint32_t*)vbase, (const uint32_t*)tbase, tris, (long long)tcap);
+  hipLaunchKernelGGL(k_mc_counts, dim3(1), dim3(64), 0, st, (const int32_t*)bv_off, (const int32_t*)bt_off,
+                     (int)nb, counts);
+  return check_launch(name);
+}
+
+// ---- marching cubes without a workspace ----------------------------------------
+// fx_mc_extract_static runs fx_mc_extract's kernels with their scratch in a
+// fixed device array instead of a caller workspace, for callers that cannot
+// allocate (code under graph capture, loops that forbid hipMalloc).  The array
+// holds the scratch of kMcStaticSamples samples - twice the reference's
+// surface cap of 2^20 samples (solver/surface.py:49).  Each GPU the library
+// runs on holds its own copy until the process ends; every call on a GPU
+// shares that copy, so calls that overlap on one GPU race.  Nothing in it
+// needs to start at zero: every kernel writes what the next one reads.
+constexpr long long kMcStaticSamples = 1LL << 21;
+constexpr size_t kMcStaticBytes = 22u << 20;  // >= mc_layout(kMcStaticSamples).total (checked)
+static __device__ __align__(256) uint8_t scratch[kMcStaticBytes];
+
+// The array's device address, looked up once per device with that device
+// current (the array's name in host code is only a stand-in for it).
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[64] = {};
+
+static int mc_static_scratch(char** out) {
+  int dev = 0;
+  FX_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(FX_EINVAL, "fx_mc_extract_static: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    if (mc_layout(kMcStaticSamples).total > kMcStaticBytes)
+      return fail(FX_ENOMEM, "fx_mc_extract_static: fixed scratch smaller than its sample limit needs");
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(FX_EHIP, "fx_mc_extract_static: hipGetSymbolAddress: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = (char*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return FX_OK;
+}
+
 }  // namespace fx
 
 using namespace fx;
@@ -832,38 +905,27 @@ size_t fx_mc_workspace_bytes(const int32_t dims[3]) {
 int fx_mc_extract(const float* field, const int32_t dims[3], const double origin[3], double spacing,
                   float iso, float* verts, int64_t vcap, int32_t* tris, int64_t tcap, int64_t* counts,
                   uint8_t* cases, void* work, size_t work_bytes, void* stream) {
-  if (!field || !dims || !origin || !counts || !work)
-    return fail(FX_EINVAL, "fx_mc_extract: null argument");
-  if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return fail(FX_EINVAL, "fx_mc_extract: dims must be >= 1");
-  if (vcap < 0 || tcap < 0) return fail(FX_EINVAL, "fx_mc_extract: negative capacity");
-  if ((vcap > 0 && !verts) || (tcap > 0 && !tris)) return fail(FX_EINVAL, "fx_mc_extract: null output");
+  if (!work) return fail(FX_EINVAL, "fx_mc_extract: null workspace");
+  return mc_extract_run("fx_mc_extract", field, dims, origin, spacing, iso, verts, vcap, tris, tcap, counts, cases,
+                        (char*)work, work_bytes, (hipStream_t)stream);
+}
+
+int fx_mc_extract_static(const float* field, const int32_t dims[3], const double origin[3], double spacing,
+                         float iso, float* verts, int64_t vcap, int32_t* tris, int64_t tcap, int64_t* counts,
+                         uint8_t* cases, void* stream) {
+  if (!dims) return fail(FX_EINVAL, "fx_mc_extract_static: null dims");
+  if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1)
+    return fail(FX_EINVAL, "fx_mc_extract_static: dims must be >= 1");
   const long long S = (long long)dims[0] * dims[1] * dims[2];
-  if (S > (1LL << 28)) return fail(FX_ERANGE, "fx_mc_extract: %lld samples exceed 2^28", S);
-  const McWs wl = mc_layout(S);
-  if (work_bytes < wl.total) return fail(FX_ENOMEM, "fx_mc_extract: workspace %zu < %zu", work_bytes, wl.total);
-  hipStream_t st = (hipStream_t)stream;
-  char* b = (char*)work;
-  uint16_t* code = (uint16_t*)(b + wl.code);
-  uint32_t* vbase = (uint32_t*)(b + wl.vbase);
-  uint32_t* tbase = (uint32_t*)(b + wl.tbase);
-  int32_t* bv = (int32_t*)(b + wl.bv);
-  int32_t* bt = (int32_t*)(b + wl.bt);
-  int32_t* bv_off = (int32_t*)(b + wl.bv_off);
-  int32_t* bt_off = (int32_t*)(b + wl.bt_off);
-  int32_t* part = (int32_t*)(b + wl.part);
-  const long long nb = (S + kMcBlock - 1) / kMcBlock;
-  McLattice L{dims[0], dims[1], dims[2], S, iso};
-  FX_LAUNCH(k_mc_classify, dim3((unsigned)nb), dim3(kMcBlock), st, L, field, code, bv, bt, cases);
-  FX_TRY(exclusive_scan_i32(bv, bv_off, nb, part, st));
-  FX_TRY(exclusive_scan_i32(bt, bt_off, nb, part, st));
-  FX_LAUNCH(k_mc_vertices, dim3((unsigned)nb), dim3(kMcBlock), st, L, field,
-                     (const uint16_t*)code, (const int32_t*)bv_off, (const int32_t*)bt_off, vbase, tbase,
-                     origin[0], origin[1], origin[2], spacing, verts, (long long)vcap);
-  FX_LAUNCH(k_mc_triangles, dim3((unsigned)nb), dim3(kMcBlock), st, L, (const uint16_t*)code,
-                     (const uint32_t*)vbase, (const uint32_t*)tbase, tris, (long long)tcap);
-  hipLaunchKernelGGL(k_mc_counts, dim3(1), dim3(64), 0, st, (const int32_t*)bv_off, (const int32_t*)bt_off,
-                     (int)nb, counts);
-  return check_launch("fx_mc_extract");
+  if (S > kMcStaticSamples)
+    return fail(FX_ERANGE,
+                "fx_mc_extract_static: %lld samples exceed its fixed scratch (%lld samples); use fx_mc_extract "
+                "with a workspace",
+                S, kMcStaticSamples);
+  char* ptr = nullptr;
+  FX_TRY(mc_static_scratch(&ptr));
+  return mc_extract_run("fx_mc_extract_static", field, dims, origin, spacing, iso, verts, vcap, tris, tcap, counts,
+                        cases, ptr, kMcStaticBytes, (hipStream_t)stream);
 }
 
 size_t fx_mc_dedup_workspace_bytes(int64_t ntris) { return ntris < 0 ? 0 : dedup_layout(ntris).total; }

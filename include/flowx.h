@@ -285,6 +285,19 @@ int fx_mc_extract(const float* field, const int32_t dims[3], const double origin
                   int64_t tcap, int64_t* counts, uint8_t* cases, void* work,
                   size_t work_bytes, void* stream);
 
+/* Marching cubes with no workspace argument: fx_mc_extract's arguments and
+ * results, its scratch in a fixed device array of the library instead of
+ * `work`.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid hipMalloc).  It allocates nothing.  Limit: dims product <= 2^21
+ * samples (twice the reference's 2^20 surface cap, solver/surface.py:49);
+ * larger lattices return FX_ERANGE.  Each GPU holds one copy of the array
+ * (about 22 MB) until the process ends, shared by every call on that GPU:
+ * calls that overlap on one GPU race. */
+int fx_mc_extract_static(const float* field, const int32_t dims[3], const double origin[3],
+                         double spacing, float iso, float* verts, int64_t vcap, int32_t* tris,
+                         int64_t tcap, int64_t* counts, uint8_t* cases, void* stream);
+
 /* Surface hand-off (solver/surface.py:278-301): the triangles bmesh would
  * accept, in order - a triangle whose vertex set (any winding) already
  * appeared earlier is dropped, as bmesh.faces.new rejects it.  tris: device
