@@ -20,9 +20,19 @@ Parity pins (what checks each function):
     position corrections is zero for mutual pairs; XSPH of a uniform
     velocity field is zero; brute-force O(N^2) neighbour sums equal the
     grid path; clamp and push-out against an analytic box; FP32 within
-    1e-5 of FP64.  Per-pass GLSL-on-Blender output itself is not
-    available anywhere (no Blender, no GL context), so agreement with the
-    reference's own executable is not pinned beyond these checks.
+    1e-5 of FP64.
+  * orc_normal: the two-particle closed form of n and curvature (the
+    neighbour's density) and an O(N^2) float64 brute force.
+  * orc_lambda: lambda itself (the gradient sums of its denominator) by
+    O(N^2) float64 brute force.
+  * orc_delta: the two-particle closed form, s_corr on and off, with and
+    without the mass weighting.
+  * orc_xsph: the two-particle closed form with its viscosity factor.
+  * orc_splat: O(N * S) float64 brute force over every particle and sample,
+    with the collider carve-out.
+    Per-pass GLSL-on-Blender output itself is not available anywhere (no
+    Blender, no GL context), so agreement with the reference's own
+    executable is not pinned beyond these checks.
 """
 
 import ctypes

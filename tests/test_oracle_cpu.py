@@ -348,3 +348,233 @@ def test_open_mesh_parity_and_restart_epsilon(fx, oracle):
     assert np.array_equal(oracle.voxelize_mesh(dims, origin, voxel, twice, tt, lo, hi, 1e-4), one)
     apart = np.concatenate([quad, quad + [0, 0, 2e-3]])
     assert oracle.voxelize_mesh(dims, origin, voxel, apart, tt, lo, hi, 1e-3).sum() == 0
+
+
+# ---- pins of the passes without a known answer above: closed forms written
+# from the SPH kernels themselves (Poly6 W = 315/(64 pi h^9) (h^2 - r^2)^3,
+# its gradient -6 * 315/(64 pi h^9) (h^2 - r^2)^2 d, the spiky gradient
+# magnitude -45/(pi h^6) (h - r)^2, the viscosity Laplacian 45/(pi h^6) (h - r))
+# and float64 brute force over all pairs, so a dropped factor, a wrong sign
+# or the wrong particle's density fails them.
+
+def _pair_params(fx, h=0.1, mass=0.37, rho0=1000.0, eps=100.0, visc=0.37, scorr=0.0, flags=0):
+    cfg = fx.lattice_config((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), h, (1, 1, 1))
+    p = cfg.params(0.01)
+    p.mass, p.rest_density, p.relaxation, p.viscosity, p.scorr_k, p.flags = mass, rho0, eps, visc, scorr, flags
+    return p
+
+
+def _pair_grid(oracle, p, pos4, n):
+    keys, idx, start, end = oracle.build_grid(p, pos4, "stable", precision="f64")
+    return idx[:n], start, end
+
+
+POLY6 = lambda h: 315.0 / (64.0 * np.pi * h**9)
+SPIKY = lambda h: -45.0 / (np.pi * h**6)
+VLAP = lambda h: 45.0 / (np.pi * h**6)
+
+
+def test_normal_two_particle_closed_form(fx, oracle):
+    """shaders/sph_normal.glsl:24-81 for two particles a distance r < h
+    apart with different densities: n_i = (m / rho_j) * gradW(x_i - x_j),
+    lap_i = (m / rho_j) * 45/(pi h^6) (h - r), curvature = -lap_i / |n_i|
+    (the neighbour's density, not the particle's own)."""
+    h = 0.1
+    p = _pair_params(fx, h=h)
+    h = float(p.h)
+    p.n = 2
+    x = np.array([[0.011, -0.02, 0.03, 1.0], [0.05, 0.012, 0.061, 1.0]])
+    lam = np.array([[950.0, 0, 0, 0], [1130.0, 0, 0, 0]])
+    idx, start, end = _pair_grid(oracle, p, x, 2)
+    out = oracle.normal(p, x, lam, idx, start, end, precision="f64")
+    for i, j in ((0, 1), (1, 0)):
+        d = x[i, :3] - x[j, :3]
+        r2 = d @ d
+        w = p.mass / lam[j, 0]
+        n = w * (-6.0 * POLY6(h) * (h * h - r2) ** 2) * d
+        lap = w * VLAP(h) * (h - np.sqrt(r2))
+        assert np.allclose(out[i, :3], n, rtol=1e-12, atol=0)
+        assert np.isclose(out[i, 3], -lap / np.linalg.norm(n), rtol=1e-12)
+    # a particle alone has no normal and no curvature
+    p.n = 1
+    idx, start, end = _pair_grid(oracle, p, x[:1], 1)
+    assert np.all(oracle.normal(p, x[:1], lam[:1], idx, start, end, precision="f64") == 0)
+
+
+def _cloud(n, seed, lo=-0.3, hi=0.3):
+    rng = np.random.default_rng(seed)
+    x = np.ones((n, 4))
+    x[:, :3] = rng.uniform(lo, hi, (n, 3))
+    return x
+
+
+def test_normal_grid_equals_brute_force(fx, oracle):
+    """The normal pass over the grid against an O(N^2) float64 sum over all
+    pairs within h, with densities that vary per particle."""
+    h = 0.1
+    p = _pair_params(fx, h=h)
+    h = float(p.h)
+    n = 1500
+    x = _cloud(n, 3)
+    p.n = n
+    rng = np.random.default_rng(4)
+    lam = np.zeros((n, 4))
+    lam[:, 0] = rng.uniform(500.0, 1500.0, n)
+    idx, start, end = _pair_grid(oracle, p, x, n)
+    out = oracle.normal(p, x, lam, idx, start, end, precision="f64")
+    d = x[:, None, :3] - x[None, :, :3]
+    r2 = np.einsum("ijk,ijk->ij", d, d)
+    near = (r2 < h * h) & ~np.eye(n, dtype=bool)
+    w = np.where(near, p.mass / lam[None, :, 0], 0.0)
+    g = -6.0 * POLY6(h) * np.where(near, (h * h - r2) ** 2, 0.0)
+    nrm = np.einsum("ij,ij,ijk->ik", w, g, d)
+    lap = (w * VLAP(h) * np.where(near, h - np.sqrt(r2), 0.0)).sum(1)
+    mag = np.linalg.norm(nrm, axis=1)
+    kappa = np.where(mag > 1e-4, -lap / np.maximum(mag, 1e-300), 0.0)
+    assert near.sum() > 20 * n
+    assert np.allclose(out[:, :3], nrm, rtol=1e-10, atol=1e-10 * np.abs(nrm).max())
+    assert np.allclose(out[:, 3], kappa, rtol=1e-9, atol=0)
+
+
+def test_lambda_grid_equals_brute_force_with_gradients(fx, oracle):
+    """lambda itself (shaders/sph_lambda.glsl:58-77): C = max(rho, 1e-6) /
+    rho0 - 1 over the full density sum, denominator (|sum_j g_ij|^2 +
+    sum_j |g_ij|^2) / rho0^2 + eps with g_ij = m * spiky(r) * d / r over
+    j != i, 1e-12 < r^2 < h^2 - brute force over all pairs in float64."""
+    h = 0.1
+    p = _pair_params(fx, h=h, mass=0.9, rho0=800.0, eps=3.0)
+    h = float(p.h)
+    n = 1200
+    x = _cloud(n, 5, -0.25, 0.25)
+    p.n = n
+    idx, start, end = _pair_grid(oracle, p, x, n)
+    out = oracle.lambda_(p, x, idx, start, end, precision="f64")
+    d = x[:, None, :3] - x[None, :, :3]
+    r2 = np.einsum("ijk,ijk->ij", d, d)
+    rho = (p.mass * POLY6(h) * np.where(r2 < h * h, (h * h - r2) ** 3, 0.0)).sum(1)
+    pair = (r2 < h * h) & (r2 > 1e-12) & ~np.eye(n, dtype=bool)
+    r = np.sqrt(np.where(pair, r2, 1.0))
+    s = np.where(pair, p.mass * SPIKY(h) * (h - r) ** 2 / r, 0.0)
+    g = s[:, :, None] * d
+    gsum = g.sum(1)
+    gn2 = np.einsum("ijk,ijk->i", g, g)
+    c = np.maximum(rho, 1e-6) / p.rest_density - 1.0
+    denom = (np.einsum("ik,ik->i", gsum, gsum) + gn2) / p.rest_density**2 + p.relaxation
+    lam = -c / np.maximum(denom, 1e-9)
+    assert np.allclose(out[:, 0], np.maximum(rho, 1e-6), rtol=1e-11)
+    assert np.allclose(out[:, 1], lam, rtol=1e-9, atol=1e-12 * np.abs(lam).max())
+    # the gradient terms matter here: with them dropped lambda would be -C / eps
+    assert np.abs(lam + c / p.relaxation).max() > 0.1 * np.abs(lam).max()
+
+
+@pytest.mark.parametrize("flags,scorr", [(0, 0.0), (0, 0.1), (1, 0.0), (1, 0.1)])
+def test_delta_two_particle_closed_form(fx, oracle, flags, scorr):
+    """shaders/sph_delta.glsl:22-85 for two particles: dp_i = (lambda_i +
+    lambda_j + s_corr) * spiky(r) * (d / r) / rho0 with s_corr =
+    -k (W(r) / W(0.2 h))^4, times the mass with FX_FLAG_MASS_WEIGHTED_DELTA."""
+    h = 0.1
+    p = _pair_params(fx, h=h, mass=0.37, rho0=1000.0, scorr=scorr, flags=flags)
+    h, scorr = float(p.h), float(p.scorr_k)
+    p.n = 2
+    x = np.array([[0.011, -0.02, 0.03, 1.0], [0.05, 0.012, 0.061, 1.0]])
+    lam = np.array([[0, -2.5e-3, 0, 0], [0, 4.0e-3, 0, 0]])
+    idx, start, end = _pair_grid(oracle, p, x, 2)
+    out = oracle.delta(p, x, lam, idx, start, end, precision="f64")
+    for i, j in ((0, 1), (1, 0)):
+        d = x[i, :3] - x[j, :3]
+        r = np.linalg.norm(d)
+        wq = POLY6(h) * (h * h - (0.2 * h) ** 2) ** 3
+        sc = -scorr * (POLY6(h) * (h * h - r * r) ** 3 / wq) ** 4
+        dp = (lam[i, 1] + lam[j, 1] + sc) * SPIKY(h) * (h - r) ** 2 * d / r / p.rest_density
+        if flags:
+            dp = dp * p.mass
+        assert np.allclose(out[i, :3], dp, rtol=1e-12, atol=0)
+
+
+def test_xsph_two_particle_closed_form(fx, oracle):
+    """shaders/sph_xsph.glsl:28-82 for two particles: dv_i = nu * (m / rho_j)
+    * (v_j - v_i) * W(r) (the viscosity factor, the neighbour's density)."""
+    h = 0.1
+    p = _pair_params(fx, h=h, visc=0.37)
+    h = float(p.h)
+    p.n = 2
+    x = np.array([[0.011, -0.02, 0.03, 1.0], [0.05, 0.012, 0.061, 1.0]])
+    v = np.array([[0.3, -0.1, 0.05, 0.0], [-0.2, 0.4, 0.1, 0.0]])
+    lam = np.array([[950.0, 0, 0, 0], [1130.0, 0, 0, 0]])
+    idx, start, end = _pair_grid(oracle, p, x, 2)
+    out = oracle.xsph(p, x, v, lam, idx, start, end, precision="f64")
+    for i, j in ((0, 1), (1, 0)):
+        d = x[i, :3] - x[j, :3]
+        w = POLY6(h) * (h * h - d @ d) ** 3
+        dv = p.viscosity * (p.mass / lam[j, 0]) * (v[j, :3] - v[i, :3]) * w
+        assert np.allclose(out[i, :3], dv, rtol=1e-12, atol=0)
+
+
+def test_splat_closed_form_brute_force_and_carve_out(fx, oracle):
+    """shaders/surface_splat.glsl:32-97: value(s) = sum_j m W_kr(|s - x_j|^2)
+    / rho0 over every particle within kr of sample s (brute force over all
+    particles and samples), 0 at samples inside an occupied collider voxel."""
+    from flowx_amd.collision import grid_geometry, occupancy_cpu
+
+    cfg = fx.lattice_config((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), 0.1, (6, 6, 4))
+    surf = fx.resolve(cfg, 24, 0.5)
+    origin, voxel, dims = grid_geometry(cfg.lo, cfg.hi, 10)
+    col = occupancy_cpu(origin, voxel, dims, [("box", (-0.15, -0.15, -0.5), (0.15, 0.15, -0.2))]).reshape(-1)
+    cfg.collider_dims, cfg.collider_voxel = dims, voxel
+    p = cfg.params(0.01)
+    x = _cloud(300, 9, -0.45, 0.45)
+    p.n = len(x)
+    idx, start, end = _pair_grid(oracle, p, x, p.n)
+    out = oracle.splat(p, surf.struct(), x, idx, start, end, col, precision="f64")
+    nx, ny, nz = surf.dims
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    s = np.stack([surf.lo[0] + i * surf.spacing, surf.lo[1] + j * surf.spacing, surf.lo[2] + k * surf.spacing],
+                 -1).reshape(-1, 3).astype(np.float64)
+    # the oracle evaluates sample positions from the float32 lattice constants
+    s = np.float32(surf.lo)[None, :].astype(np.float64) + np.stack([i, j, k], -1).reshape(-1, 3) * float(
+        np.float32(surf.spacing))
+    kr = float(np.float32(surf.kernel_radius))
+    d = s[:, None, :] - x[None, :, :3]
+    r2 = np.einsum("ijk,ijk->ij", d, d)
+    val = (p.mass * POLY6(kr) * np.where(r2 < kr * kr, (kr * kr - r2) ** 3, 0.0)).sum(1) / p.rest_density
+    vi = np.floor((s - np.asarray(cfg.lo)) / float(np.float32(voxel))).astype(np.int64)
+    inside = np.all((vi >= 0) & (vi < np.asarray(dims)), axis=1)
+    occ = np.zeros(len(s), bool)
+    occ[inside] = col.reshape(dims[2], dims[1], dims[0])[vi[inside, 2], vi[inside, 1], vi[inside, 0]] > 0
+    val[occ] = 0.0
+    assert occ.sum() > 0 and (val > 0).sum() > 1000
+    assert np.allclose(out, val, rtol=1e-10, atol=1e-12 * val.max())
+
+
+# ---- configuration and substep sizing (solver/sph.py:228-313) ----------------
+
+def test_substep_sizing_and_capped_grids_match_the_survey_numbers(fx):
+    """Substeps per frame and dt at 24 fps for the benchmark configurations,
+    the cell grids the reference's caps (MAX_CELLS 262144, 128 per axis)
+    give at h = 0.1 / 0.025 / 0.018 (cells grown 1.5x until they fit: 20^3,
+    54^3 at 1.5 h, 50^3 at 2.25 h), and the particle-count coarsening of the
+    default domain (39 x 39 x 18 = 27,378 > 16,384 seeds - in float64
+    (1.0 - 0.05) / 0.05 truncates to 18 - so spacing x (27378/16384)^(1/3),
+    32 x 32 x 15 = 15,360; at 40% fill 39 x 39 x 15 = 22,815 -> 34 x 34 x 13 =
+    15,028)."""
+    from flowx_amd.config import REFERENCE_CAPS, DomainSettings, _cell_grid, resolve_config
+
+    expect = {"dam_break_config": (2, 1 / 48), "scale_config": (4, 1 / 96), "adjoint_config": (3, 1 / 72),
+              "animated_config": (4, 1 / 96)}
+    for name, (steps, dt) in expect.items():
+        s, d = getattr(fx, name)().substep_dt(1.0 / 24.0)
+        assert s == steps and abs(d - dt) < 1e-15, name
+    size = (2.0, 2.0, 2.0)
+    for h, dims, ratio in ((0.1, 20, 1.0), (0.025, 54, 1.5), (0.018, 50, 2.25), (0.05, 40, 1.0)):
+        cell, d = _cell_grid(size, h, REFERENCE_CAPS)
+        assert d == (dims,) * 3 and abs(cell / h - ratio) < 1e-12, h
+    c = resolve_config(DomainSettings(), (-1, -1, 0), (1, 1, 2), REFERENCE_CAPS)
+    assert c.particle_count == 15360 and abs(c.spacing - 0.05 * (27378 / 16384) ** (1 / 3)) < 1e-12
+    assert abs(c.smoothing_radius - 2 * c.spacing) < 1e-15 and abs(c.mass - 1000 * c.spacing**3) < 1e-12
+    c = resolve_config(DomainSettings(fluid_level=40.0), (-1, -1, 0), (1, 1, 2), REFERENCE_CAPS)
+    assert c.particle_count == 34 * 34 * 13
+    # the viscous limit takes over for a thick fluid: 0.125 h^2 / (nu / rho0)
+    thick = resolve_config(DomainSettings(viscosity=10.0), (-1, -1, 0), (1, 1, 2), REFERENCE_CAPS)
+    s, d = thick.substep_dt(1.0 / 24.0)
+    limit = min(0.25 * np.sqrt(thick.smoothing_radius / 9.81), 0.125 * thick.smoothing_radius**2 / (10.0 / 1000.0))
+    assert s == min(max(1, int(np.ceil((1 / 24) / limit))), 24) and abs(d - min((1 / 24) / s, limit)) < 1e-15
