@@ -231,6 +231,50 @@ __device__ __forceinline__ void quad_run(int s, int e, int j, LD&& ld, TM&& tm, 
   }
 }
 
+// ---- block-aggregated counting-sort ranks ---------------------------------------
+// The counting sort's rank of a particle in its cell is its arrival order at
+// the cell's counter.  One returning global atomic per particle made that the
+// larger part of the fused finalize + predict pass (84 us at 1.64M particles,
+// 29 us with the atomic taken out).  Here the threads of a block first gather
+// their keys in an LDS hash table (one entry per distinct key, its count
+// taken with LDS atomics); one thread per distinct key then adds the block's
+// count to the global counter and gets the block's base back, and a thread's
+// rank is that base plus its arrival order in the block.  Particles in slot
+// or lattice order give a block of 256 some 30-80 distinct cells, so the
+// global atomics drop several-fold.  Ranks within a cell stay in arrival
+// order: the canonicalisation after the scatter orders each cell's run.
+// Every thread of the block must call it (barriers); `active` false for
+// threads past the end.  Keys are >= 0.
+template <int BLOCK>
+__device__ __forceinline__ int block_rank(int key, bool active, int32_t* __restrict__ cnt) {
+  constexpr int H = 2 * BLOCK;  // hash slots: at most BLOCK distinct keys
+  static_assert((H & (H - 1)) == 0, "power of two");
+  __shared__ int hkey[H];
+  __shared__ int hval[H];  // the block's count of the key, then its base
+  for (int i = threadIdx.x; i < H; i += BLOCK) {
+    hkey[i] = -1;
+    hval[i] = 0;
+  }
+  __syncthreads();
+  int slot = 0, local = 0;
+  if (active) {
+    slot = (int)(((unsigned)key * 2654435761u) >> 16) & (H - 1);
+    for (int probe = 0; probe < H; ++probe) {  // ends: fewer keys than slots
+      const int prev = atomicCAS(&hkey[slot], -1, key);
+      if (prev == -1 || prev == key) break;
+      slot = (slot + 1) & (H - 1);
+    }
+    local = atomicAdd(&hval[slot], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < H; i += BLOCK) {
+    const int c = hval[i];
+    if (c > 0) hval[i] = atomicAdd(&cnt[hkey[i]], c);
+  }
+  __syncthreads();
+  return hval[slot] + local;
+}
+
 // ---- launch helpers ---------------------------------------------------------
 constexpr int kBlock = 256;
 

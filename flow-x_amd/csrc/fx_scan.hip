@@ -170,16 +170,16 @@ __global__ __launch_bounds__(1024) void k_scan_small(const int32_t* __restrict__
 int exclusive_scan_i32(const int32_t* counts, int32_t* out, long long n, int32_t* partial,
                        hipStream_t st) {
   if (n <= kSmallScanMax) {
-    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, st, counts, out, (int)n);
+    FX_LAUNCH(k_scan_small, dim3(1), dim3(1024), st, counts, out, (int)n);
     return check_launch("exclusive_scan_i32");
   }
   long long nb = (n + kScanTile - 1) / kScanTile;
   if (nb < 1) nb = 1;
   if (nb > kMaxPartials) return fail(FX_ERANGE, "scan of %lld items exceeds %d tiles", n, kMaxPartials);
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, st, counts, n, partial);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, partial, (int)nb);
-  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(kScanThreads), 0, st, counts, out, n,
-                     (const int32_t*)partial, (int)nb);
+  FX_LAUNCH(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), st, counts, n, partial);
+  FX_LAUNCH(k_scan_partials, dim3(1), dim3(1024), st, partial, (int)nb);
+  FX_LAUNCH(k_scan_final, dim3((unsigned)nb), dim3(kScanThreads), st, counts, out, n, (const int32_t*)partial,
+            (int)nb);
   return check_launch("exclusive_scan_i32");
 }
 

@@ -351,23 +351,29 @@ __global__ __launch_bounds__(256) void k_predict_key(FxParams p, Grid g, const f
                                                      int32_t* __restrict__ cnt,
                                                      int32_t* __restrict__ rank, int mode) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.n) return;
-  float4 pp;
-  if (mode & 1) {
-    float4 vo;
-    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-    predict_one(p, pos[i], vel[i], (p.surface_tension > 0.0f && nrm) ? nrm[i] : zero, vo, pp);
-    vel[i] = vo;
-    pred[i] = pp;
-  } else {
-    pp = pred[i];
+  const bool active = i < p.n;
+  int k = 0;
+  if (active) {
+    float4 pp;
+    if (mode & 1) {
+      float4 vo;
+      const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+      predict_one(p, pos[i], vel[i], (p.surface_tension > 0.0f && nrm) ? nrm[i] : zero, vo, pp);
+      vel[i] = vo;
+      pred[i] = pp;
+    } else {
+      pp = pred[i];
+    }
+    if (mode & 6) {
+      int x, y, z;
+      cell_coord(g, pp.x, pp.y, pp.z, x, y, z);
+      k = cell_index(g, x, y, z);
+      key[i] = k;
+    }
   }
-  if (mode & 6) {
-    int x, y, z;
-    cell_coord(g, pp.x, pp.y, pp.z, x, y, z);
-    const int k = cell_index(g, x, y, z);
-    key[i] = k;
-    if (mode & 2) rank[i] = atomicAdd(&cnt[k], 1);
+  if (mode & 2) {  // uniform over the launch: every thread reaches the barriers
+    const int r = block_rank<kBlock>(k, active, cnt);
+    if (active) rank[i] = r;
   }
 }
 
@@ -433,23 +439,27 @@ __global__ __launch_bounds__(256) void k_finalize_predict(int n, FxParams p, Col
                                                           int32_t* __restrict__ rank,
                                                           float4* __restrict__ nrm_canon) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const float4 pi = pred_s[t], vi = vel_s[t], dv = dv_s[t];
-  float vx = vi.x + dv.x, vy = vi.y + dv.y, vz = vi.z + dv.z;
-  float px = pi.x, py = pi.y, pz = pi.z;
-  finalize_one(p, col, px, py, pz, vx, vy, vz);
-  const float4 pf = make_float4(px, py, pz, 1.0f), vf = make_float4(vx, vy, vz, 0.0f);
-  pos_r[t] = pf;
-  const float4 nrm = (p.surface_tension > 0.0f && nrm_r) ? nrm_r[t] : make_float4(0.f, 0.f, 0.f, 0.f);
-  float4 vo, pp;
-  predict_one(p, pf, vf, nrm, vo, pp);
-  pred_r[t] = pp;
-  int x, y, z;
-  cell_coord(g, pp.x, pp.y, pp.z, x, y, z);
-  const int k = cell_index(g, x, y, z);
-  key[t] = k;
-  rank[t] = atomicAdd(&cnt[k], 1);
-  if (nrm_canon) nrm_canon[min(max(sidx[t], 0), n - 1)] = nrm;
+  const bool active = t < n;
+  int k = 0;
+  if (active) {
+    const float4 pi = pred_s[t], vi = vel_s[t], dv = dv_s[t];
+    float vx = vi.x + dv.x, vy = vi.y + dv.y, vz = vi.z + dv.z;
+    float px = pi.x, py = pi.y, pz = pi.z;
+    finalize_one(p, col, px, py, pz, vx, vy, vz);
+    const float4 pf = make_float4(px, py, pz, 1.0f), vf = make_float4(vx, vy, vz, 0.0f);
+    pos_r[t] = pf;
+    const float4 nrm = (p.surface_tension > 0.0f && nrm_r) ? nrm_r[t] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 vo, pp;
+    predict_one(p, pf, vf, nrm, vo, pp);
+    pred_r[t] = pp;
+    int x, y, z;
+    cell_coord(g, pp.x, pp.y, pp.z, x, y, z);
+    k = cell_index(g, x, y, z);
+    key[t] = k;
+    if (nrm_canon) nrm_canon[min(max(sidx[t], 0), n - 1)] = nrm;
+  }
+  const int r = block_rank<kBlock>(k, active, cnt);
+  if (active) rank[t] = r;
 }
 
 // Counting-sort scatter of (particle << 32 | previous slot) pairs.  The
@@ -1640,25 +1650,29 @@ __global__ __launch_bounds__(256) void k_b_predict_key(int ntot, Scenes S, const
                                                        int32_t* __restrict__ cnt,
                                                        int32_t* __restrict__ rank, int do_predict) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ntot) return;
-  const SceneDev& sc = S.s[scene_of(S, i)];
-  float4 pp;
-  if (do_predict) {
-    float4 vo;
-    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-    predict_one(sc.p, pos[i], vel[i], (sc.p.surface_tension > 0.0f && nrm) ? nrm[i] : zero, vo, pp);
-    vel[i] = vo;
-    pred[i] = pp;
-  } else {
-    pp = pos[i];
-    pp.w = 1.0f;
-    pred[i] = pp;
+  const bool active = i < ntot;
+  int k = 0;
+  if (active) {
+    const SceneDev& sc = S.s[scene_of(S, i)];
+    float4 pp;
+    if (do_predict) {
+      float4 vo;
+      const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+      predict_one(sc.p, pos[i], vel[i], (sc.p.surface_tension > 0.0f && nrm) ? nrm[i] : zero, vo, pp);
+      vel[i] = vo;
+      pred[i] = pp;
+    } else {
+      pp = pos[i];
+      pp.w = 1.0f;
+      pred[i] = pp;
+    }
+    int x, y, z;
+    cell_coord(sc.g, pp.x, pp.y, pp.z, x, y, z);
+    k = sc.coff + cell_index(sc.g, x, y, z);
+    key[i] = k;
   }
-  int x, y, z;
-  cell_coord(sc.g, pp.x, pp.y, pp.z, x, y, z);
-  const int k = sc.coff + cell_index(sc.g, x, y, z);
-  key[i] = k;
-  rank[i] = atomicAdd(&cnt[k], 1);
+  const int r = block_rank<kBlock>(k, active, cnt);
+  if (active) rank[i] = r;
 }
 
 __global__ void k_b_load_state(int ntot, float4* __restrict__ pos, float4* __restrict__ vel) {
