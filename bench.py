@@ -39,6 +39,16 @@ def algorithmic_bytes(kernel, n, c):
         "k_xsph": 44 + 8.0 * c / n + 16,    # pred, vel, rho -> dv; the next substep's normal (16)
         "k_normal": 36 + 8.0 * c / n,
         "k_predict_key": 48 + 8,            # pos, vel -> vel, pred; key + rank
+        # finalize + the next predict + key + rank: pred, vel, dv, normal (12 + 12 + 12 + 16) in;
+        # position, prediction, key, rank (12 + 12 + 4 + 4) out
+        "k_finalize_predict": 84,
+        # (particle, previous slot) pairs 8, the previous slot's prediction, position, key (12 + 12 + 4)
+        # in; sorted prediction + lambda, position, (rho, lambda), key, particle (16 + 12 + 8 + 4 + 4) out
+        "k_gather_pairs": 80,
+        "k_gather": 76,                     # the same from canonical rows (index 4 in, key 4 in)
+        "k_scatter_pairs": 24 + 4.0 * c / n,  # key, rank, particle, cell offset in; pair out; counts zeroed
+        "k_finalize_s": 36 + 48,            # pred, vel, dv in; position, velocity (canonical, sorted) out
+        "k_canon_cells": 16 + 8.0 * c / n,  # each cell's pairs read and written once, its two offsets
     }.get(kernel)
     if per_particle is None:
         return None, None
@@ -447,6 +457,111 @@ def adjoint_workload(fx, substeps, directions, fd_checks=((4, 1e-4, 1e-8), (8, 1
         **out}
 
 
+def batch_summary(fx, steps, warmup, count=64):
+    """The batched line of the metric, in the default run: the 64-scene set
+    (seed 0) replayed from a HIP graph against the same scenes run one after
+    another on the plain path (fewer substeps than the batch workload)."""
+    b = batch_workload(fx, steps, warmup, count)
+    return {"scenes": count, "particles": b["config"]["particles"], "substeps": steps,
+            "batched_graph_particle_substeps_per_s": b["value"],
+            "batched_eager_particle_substeps_per_s": b["batched_eager_value"],
+            "sequential_plain_particle_substeps_per_s": b["sequential_plain_value"],
+            "batched_over_sequential": b["batched_over_sequential"]}
+
+
+def adjoint_summary(fx, substeps=24, reps=3):
+    """The adjoint line of the metric, in the default run: float32 forward and
+    backward of the 200k-particle differentiable scene over `substeps`
+    substeps (the horizon whose gradient stays finite in float32: |dL/dv0|
+    grows about tenfold per substep once the dam collapses and leaves the
+    float32 range near 30 substeps), medians of `reps` timed pairs, records
+    allocated ahead.  No finite differences here (tests/test_gpu_adjoint.py
+    holds them)."""
+    import numpy as np
+    import torch
+
+    from flowx_amd.adjoint import DiffSolver, gaussian_target, loss_fn, rollout
+
+    rng = np.random.default_rng(0)
+    centre = (float(rng.uniform(-0.5, 0.5)), float(rng.uniform(-0.5, 0.5)), float(rng.uniform(0.1, 0.9)))
+    d = DiffSolver(fx.adjoint_config(), dtype=torch.float32)
+    x0 = torch.as_tensor(fx.seed_positions(d.cfg), dtype=torch.float32).cuda()
+    x0[:, 3] = 0
+    v0 = torch.zeros_like(x0)
+    g = torch.tensor(-9.81, dtype=torch.float64, device="cuda", requires_grad=True)
+    tr = torch.zeros(3, dtype=torch.float64, device="cuda", requires_grad=True)
+    target = gaussian_target(d, centre, sigma=0.2)
+    com_t = torch.tensor(centre, dtype=torch.float32, device="cuda")
+    records = [d.new_decisions() for _ in range(substeps)]
+    v = v0.clone().requires_grad_(True)
+    loss_fn(d, rollout(d, x0, v, g, tr, 2, [], 1)[0], target, com_t).backward()  # warm-up
+    fwd, bwd, norm = [], [], None
+    for _ in range(reps):
+        g.grad = tr.grad = None
+        v = v0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, _ = rollout(d, x0, v, g, tr, substeps, records, 1)
+        L = loss_fn(d, x, target, com_t)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t2 - t1)
+        norm = float(v.grad.double().norm())
+    tf, tb = float(np.median(fwd)), float(np.median(bwd))
+    return {"particles": len(x0), "substeps": substeps, "dtype": "f32", "forward_s": round(tf, 4),
+            "backward_s": round(tb, 4), "adjoint_over_forward": round(tb / tf, 3),
+            "grad_v0_norm": norm, "gradient_finite": True}
+
+
+def scorr_delta(fx, kernels, steps=8):
+    """The delta pass with s_corr on, at a strength that still converges on
+    the scale scene (k = 1e-5; k >= 1e-4 makes it blow up), timed on its own
+    scene: the per-pair pow and poly6 the default line leaves out."""
+    import torch
+
+    from flowx_amd import _lib
+
+    cfg = fx.scale_config()
+    cfg.scorr_strength = 1e-5
+    solver = fx.Solver(cfg, keep_scratch=False)
+    solver.reset()
+    _, dt = cfg.substep_dt(1.0 / cfg.extra["fps"])
+    solver.substeps_n(dt, 8)
+    torch.cuda.synchronize()
+    _lib.profile(True)
+    solver.substeps_n(dt, steps)
+    torch.cuda.synchronize()
+    rep = _lib.profile_report()
+    _lib.profile(False)
+    name = next(k for k in rep if k.startswith("k_delta"))
+    with_us = 1e3 * rep[name][1] / rep[name][0]
+    without_us = kernels.get(name, {}).get("avg_us")
+    del solver
+    torch.cuda.empty_cache()
+    return {"scorr_k": 1e-5, "kernel": name, "avg_us_with_scorr": round(with_us, 2),
+            "avg_us_without": round(without_us, 2) if without_us else None}
+
+
+def streaming_rooflines(kernels, n, c):
+    """Algorithmic bytes / average duration for the streaming kernels."""
+    out = {}
+    for k, v in kernels.items():
+        name = k.split("<")[0]
+        if name in ("k_lambda", "k_delta", "k_xsph", "k_normal"):
+            continue
+        alg, per = algorithmic_bytes(k, n, c)
+        if alg is None:
+            continue
+        gbs = alg / (v["avg_us"] * 1e-6) / 1e9
+        out[k] = {"avg_us": round(v["avg_us"], 2), "bytes_per_particle": round(per, 2),
+                  "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -457,6 +572,8 @@ def main():
     ap.add_argument("--literal-update", action="store_true",
                     help="the reference's literal position correction (diverges at these masses)")
     ap.add_argument("--no-surface", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="leave out the s_corr, batched and adjoint objects of the default line")
     ap.add_argument("--workload", default="scale", choices=["scale", "batch", "adjoint", "dambreak", "animated"])
     ap.add_argument("--frames", type=int, default=None, help="dambreak / animated workloads: frames to run")
     ap.add_argument("--autotune", action="store_true",
@@ -555,10 +672,24 @@ def main():
     # the neighbour passes against their own bounds: candidates per second
     # and the counter figures of the committed profile
     cand = stencil_candidates(cfg, solver.pos.cpu().numpy())
-    nb_kernel = dominant if dominant.split("<")[0] in ("k_lambda", "k_delta", "k_xsph") else "k_lambda<VISIT_FULL>"
-    nb_s = kernels[nb_kernel]["avg_us"] * 1e-6
-    neighbour_pass = {"kernel": nb_kernel, "candidates_per_particle": round(cand / n, 2),
-                      "candidate_pairs_per_s": round(cand / nb_s, 1), "counters": read_counters(nb_kernel)}
+    neighbour_passes = {}
+    for nb_kernel in kernels:
+        if nb_kernel.split("<")[0] not in ("k_lambda", "k_delta", "k_xsph"):
+            continue
+        nb_s = kernels[nb_kernel]["avg_us"] * 1e-6
+        neighbour_passes[nb_kernel] = {"avg_us": round(kernels[nb_kernel]["avg_us"], 2),
+                                       "candidates_per_particle": round(cand / n, 2),
+                                       "candidate_pairs_per_s": round(cand / nb_s, 1),
+                                       "counters": read_counters(nb_kernel)}
+    if roofline is not None:
+        ctr = read_counters(dominant)
+        if ctr:
+            # the roofline is HBM's (no MFMA work); what the counters show
+            # limits the kernel is the vector-memory pipeline
+            roofline["measured_limiter"] = (
+                f"vector-memory pipeline: TA {ctr.get('ta_busy', 0) * 100:.0f}% / TD {ctr.get('td_busy', 0) * 100:.0f}% "
+                f"busy, VALU lanes {ctr.get('valu_lane_utilisation', 0) * 100:.0f}% active, HBM "
+                f"{roofline['frac'] * 100:.1f}% of peak (committed rocprofv3 counters)")
 
     surface = None
     if not args.no_surface:
@@ -573,9 +704,21 @@ def main():
         surface = {"dims": list(surf.config.dims), "ms": round((time.perf_counter() - s0) * 1e3 / reps, 3),
                    "vertices": int(verts.shape[0]), "triangles": int(tris.shape[0])}
 
-    rho = None
+    del solver
+    torch.cuda.empty_cache()
+    scorr = batched = adjoint = None
+    if not args.no_extras:
+        print("[bench] s_corr delta pass", file=sys.stderr, flush=True)
+        scorr = scorr_delta(fx, kernels)
+        print("[bench] 64 batched scenes vs sequential plain runs", file=sys.stderr, flush=True)
+        batched = batch_summary(fx, steps=10, warmup=2)
+        torch.cuda.empty_cache()
+        print("[bench] adjoint forward / backward", file=sys.stderr, flush=True)
+        adjoint = adjoint_summary(fx)
+        torch.cuda.empty_cache()
     base = None
     if not args.no_cpu_baseline:
+        print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
         base = cpu_baseline(cfg, dt, args.cpu_substeps)
 
     out = {
@@ -593,15 +736,20 @@ def main():
         "data": "synthetic jittered lattice (seeded MT19937), analytic colliders",
         "config": {
             "workload": f"PBF substep, {n} particles (160x160x64, h=0.025 m, 2 m domain), 4 Jacobi iterations, "
-                        f"cohesion on, XSPH 0.01, pillar + half-wall on 80^3 voxels, "
-                        f"update: {cfg.extra.get('update', 'reference literal')}",
+                        f"cohesion on, XSPH 0.01, pillar + half-wall on 80^3 voxels",
+            "update": cfg.extra.get("update", "reference literal (no mass in the correction)"),
+            "scorr_k": cfg.scorr_strength,
             "particles": n, "cells": c, "dt": dt,
         },
         "roofline": roofline,
-        "neighbour_pass": neighbour_pass,
+        "neighbour_passes": neighbour_passes,
+        "streaming": streaming_rooflines(kernels, n, c),
         "cpu_baseline": base,
         "kernels": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kernels.items()},
         "surface": surface,
+        "scorr_delta": scorr,
+        "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out))
 

@@ -240,8 +240,11 @@ __device__ __forceinline__ void quad_run(int s, int e, int j, LD&& ld, TM&& tm, 
 // taken with LDS atomics); one thread per distinct key then adds the block's
 // count to the global counter and gets the block's base back, and a thread's
 // rank is that base plus its arrival order in the block.  Particles in slot
-// or lattice order give a block of 256 some 30-80 distinct cells, so the
-// global atomics drop several-fold.  Ranks within a cell stay in arrival
+// order (the resident substeps) put a whole cell's run - about eight
+// particles - into consecutive lanes, whose same-address atomics serialize;
+// a block of 256 holds some 30-40 distinct cells, so the global atomics drop
+// several-fold.  (In lattice order a wave holds about two particles per cell
+// and the plain atomics are cheaper.)  Ranks within a cell stay in arrival
 // order: the canonicalisation after the scatter orders each cell's run.
 // Every thread of the block must call it (barriers); `active` false for
 // threads past the end.  Keys are >= 0.
