@@ -1227,7 +1227,8 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
                                                 const int32_t* __restrict__ off_,
                                                 float2* __restrict__ dl_s,
                                                 const int32_t* __restrict__ sidx,
-                                                float4* __restrict__ lambda4, Lists L, Scenes S) {
+                                                float4* __restrict__ lambda4, Lists L, Scenes S,
+                                                int iter = -1) {
   constexpr int MODE = V & 3;
   constexpr bool QUAD = (V & kQuadWalk) != 0;
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1238,6 +1239,9 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
   const int32_t* off = off_;
   if (V & 8) {
     const SceneDev& sc = S.s[scene_of(S, t)];
+    // batched: a scene past its own iteration count keeps its last lambda
+    // pass (its densities feed XSPH)
+    if (iter >= sc.p.iterations) return;
     g = sc.g;
     k = sc.k;
     off = off_ + sc.coff;
@@ -1300,7 +1304,8 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
                                                float4* __restrict__ vel_s,
                                                const int32_t* __restrict__ sidx,
                                                float4* __restrict__ delta4, Lists L,
-                                               const int32_t* __restrict__ skey, Scenes S) {
+                                               const int32_t* __restrict__ skey, Scenes S,
+                                               int iter = -1) {
   constexpr int MODE = V & 3;
   constexpr bool QUAD = (V & kQuadWalk) != 0;
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1311,6 +1316,22 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
     const int32_t* off = off_;
     if (V & 8) {
       const SceneDev& sc = S.s[scene_of(S, t)];
+      const int its = sc.p.iterations;
+      if (iter >= its) {
+        // batched, the scene past its own iteration count: carry its
+        // prediction through the ping-pong unchanged (w = its last density);
+        // with no iterations at all, the velocity pass as k_velocity_s does
+        const float4 pi = pred_in[t];
+        const float2 dl = dl_s[t];
+        pred_out[t] = make_float4(pi.x, pi.y, pi.z, dl.x);
+        if (its == 0 && iter == 0) {
+          const float dtc = fmaxf(sc.k.dt, 1e-6f);
+          const float4 po = pos_s[t];
+          vel_s[t] = make_float4((pi.x - po.x) / dtc, (pi.y - po.y) / dtc, (pi.z - po.z) / dtc, dl.x);
+        }
+        return;
+      }
+      if (iter != its - 1) vel_s = nullptr;  // its last iteration writes the velocity
       g = sc.g;
       k = sc.k;
       off = off_ + sc.coff;
@@ -2202,14 +2223,12 @@ static int batch_plan(const FxScene* scenes, int count, BatchPlan& b) {
   b.ntot = 0;
   b.ctot = 0;
   b.tension = false;
-  b.iterations = scenes[0].params.iterations;
+  b.iterations = 0;  // the most any scene runs; the others carry their state through the rest
   for (int i = 0; i < count; ++i) {
     const FxParams& p = scenes[i].params;
     FxState dummy{};
     FX_TRY(validate(&p, &dummy, false));
-    if (p.iterations != b.iterations)
-      return fail(FX_EINVAL, "batch: scene %d has %d iterations, scene 0 has %d (must match)", i,
-                  p.iterations, b.iterations);
+    b.iterations = std::max(b.iterations, p.iterations);
     if (p.surface_tension > 0.0f) b.tension = true;
     b.ntot += p.n;
     b.ctot += p.n_cells;
@@ -2298,15 +2317,21 @@ static int batch_step(const FxScene* scenes, int count, float* pos, float* vel, 
   if (load || n == 0) return check_launch("batch");
   int cur = cur0;
   for (int it = 0; it < iters; ++it) {
-    const bool last = it == iters - 1;
     NLAUNCH(k_lambda<8 + VISIT_FULL>, n, st, n, Grid{}, Consts{}, w.pred_s[cur], w.off, w.dl_s, w.sidx,
-            (float4*)nullptr, L0, S);
+            (float4*)nullptr, L0, S, it);
+    // each scene writes its velocity on its own last iteration
     NLAUNCH(k_delta<8 + VISIT_FULL>, n, st, n, Grid{}, Consts{}, (const float4*)w.pred_s[cur],
-            (const float2*)w.dl_s, w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s,
-            last ? w.vel_s : (float4*)nullptr, w.sidx, (float4*)nullptr, L0, (const int32_t*)nullptr, S);
+            (const float2*)w.dl_s, w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s, w.vel_s, w.sidx,
+            (float4*)nullptr, L0, (const int32_t*)nullptr, S, it);
     cur ^= 1;
   }
-  if (iters == 0) LAUNCH(k_velocity_s, n, st, n, scenes[0].params.dt, w.pred_s[0], w.pos_s, w.dl_s, w.vel_s);
+  // no scene iterates: the batched delta launch as iteration 0 writes each
+  // scene's velocity with its own dt (its copy into pred_s[1] is overwritten
+  // by XSPH's correction below)
+  if (iters == 0)
+    NLAUNCH(k_delta<8 + VISIT_FULL>, n, st, n, Grid{}, Consts{}, (const float4*)w.pred_s[0], (const float2*)w.dl_s,
+            w.off, w.pred_s[1], (const float4*)w.pos_s, w.vel_s, w.sidx, (float4*)nullptr, L0,
+            (const int32_t*)nullptr, S, 0);
   // XSPH leaves dv by slot (in pred_s[1], free after the iterations) and the
   // finalize runs on its own, as in the plain substep: without the collider
   // search the neighbour pass holds fewer registers (91 -> 73 VGPRs)

@@ -205,8 +205,11 @@ int fx_load_state(const FxParams* p, FxState* s, void* stream);
  * launch per pass: particles are concatenated in scene order (scene i owns
  * the next scenes[i].params.n rows of pos / vel), cells are offset into one
  * key space, one counting sort orders every scene, and each thread reads its
- * scene's parameters from a table the library keeps in `work`.  Every scene
- * must use the same iteration count.  fx_batch_load_state uploads the table
+ * scene's parameters from a table the library keeps in `work`.  Scenes may
+ * differ in everything, iteration count included: the Jacobi launches run
+ * as many times as the scene with the most iterations needs, and a scene
+ * past its own count carries its state through the rest unchanged (results
+ * equal the scene's own plain substep).  fx_batch_load_state uploads the table
  * (synchronously) and builds the grid from pos (pred = pos); call it again
  * whenever a scene's parameters change.  fx_batch_substep reads only device
  * memory and may be captured in a HIP graph. */

@@ -45,6 +45,42 @@ def test_batch_equals_plain_per_scene(fx):
         assert np.array_equal(batch.scene_velocities(i), s.read_velocities()), f"scene {i} velocities"
 
 
+def test_batch_with_per_scene_iteration_counts(fx):
+    """Scenes with 4, 2, 0, 6 and 3 Jacobi iterations (and their own dt) in
+    one batch: the Jacobi launches run six times, a scene past its own count
+    carries its state through unchanged, and every scene equals its own
+    plain run bit for bit."""
+    import torch
+
+    from flowx_amd.config import converging_update
+
+    def scenes():
+        out = _scenes(fx)
+        d = converging_update(fx.lattice_config((-0.3, -0.3, 0.0), (0.3, 0.3, 0.6), 0.05, (11, 11, 8), iterations=6))
+        e = converging_update(fx.lattice_config((-0.5, -0.5, 0.0), (0.5, 0.5, 1.0), 0.07, (13, 13, 6), iterations=3,
+                                                viscosity=0.3))
+        e.extra.update(colliders=[("box", (0.0, -0.1, 0.0), (0.2, 0.1, 0.2))], collider_resolution=12)
+        out[1].iterations = 2
+        out[2].iterations = 0
+        return out + [d, e]
+
+    cfgs = scenes()
+    assert [c.iterations for c in cfgs] == [4, 2, 0, 6, 3]
+    batch = fx.BatchSolver(cfgs)
+    steps = 5
+    for _ in range(steps):
+        batch.substep()
+    torch.cuda.synchronize()
+    for i, cfg in enumerate(scenes()):
+        s = fx.Solver(cfg, keep_scratch=False)
+        s.reset()
+        _, dt = cfg.substep_dt(1.0 / 24.0)
+        s.substeps_n(dt, steps)
+        torch.cuda.synchronize()
+        assert np.array_equal(batch.scene_positions(i), s.read_positions()), f"scene {i} positions"
+        assert np.array_equal(batch.scene_velocities(i), s.read_velocities()), f"scene {i} velocities"
+
+
 @pytest.mark.parametrize("lattice", [(50, 50, 20), (70, 68, 20), (80, 80, 32)])
 def test_walks_by_scene_size_are_bit_identical(fx, lattice):
     """A plain scene picks its neighbour walk by size (four lanes per particle
