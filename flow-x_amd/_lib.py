@@ -81,7 +81,8 @@ class FxSurface(ctypes.Structure):
 
 
 class FxTuning(ctypes.Structure):
-    _fields_ = [("list_skin", ctypes.c_float), ("block", ctypes.c_int32)]
+    _fields_ = [("list_skin", ctypes.c_float), ("block", ctypes.c_int32), ("persistent", ctypes.c_int32),
+                ("walk", ctypes.c_int32)]
 
 
 class FxScene(ctypes.Structure):
@@ -216,20 +217,24 @@ def call(name, *args):
     check(getattr(lib(), name)(*args), name)
 
 
-def set_tuning(list_skin=None, block=None):
+def set_tuning(list_skin=None, block=None, persistent=None, walk=None):
     t = FxTuning()
     check(lib().fx_get_tuning(ctypes.byref(t)), "fx_get_tuning")
     if list_skin is not None:
         t.list_skin = list_skin
     if block is not None:
         t.block = block
+    if persistent is not None:
+        t.persistent = persistent
+    if walk is not None:
+        t.walk = walk
     check(lib().fx_set_tuning(ctypes.byref(t)), "fx_set_tuning")
 
 
 def get_tuning():
     t = FxTuning()
     check(lib().fx_get_tuning(ctypes.byref(t)), "fx_get_tuning")
-    return {"list_skin": t.list_skin, "block": t.block}
+    return {"list_skin": t.list_skin, "block": t.block, "persistent": t.persistent, "walk": t.walk}
 
 
 def profile(enable):

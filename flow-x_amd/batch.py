@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, autotune
 from ._lib import FxScene
 from .collision import ColliderGrid
 from .config import seed_positions
@@ -25,6 +25,7 @@ class BatchSolver:
         if not torch.cuda.is_available():
             raise RuntimeError("BatchSolver needs a ROCm GPU")
         _lib.lib()
+        autotune.apply()  # tuning.json from a previous autotune run, if any
         self.device = torch.device(device)
         self.configs = list(configs)
         self.fps = fps

@@ -333,6 +333,8 @@ Workspace workspace_layout(const FxParams& p);
 struct Tuning {
   float skin;
   int block;
+  int persistent;  // batched passes: workgroups per CU of a persistent launch, 0 = flat
+  int walk;        // large-scene neighbour passes: 0 default walk, 1 row by row, 2 the other stream walk
 };
 Tuning tuning();
 

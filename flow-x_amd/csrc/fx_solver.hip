@@ -83,7 +83,7 @@ ProfScope::~ProfScope() {
   (void)hipEventRecord(g_prof_recs[slot].b, st);
 }
 
-static Tuning g_tuning = {0.0f, 64};
+static Tuning g_tuning = {0.0f, 64, 0, 0};
 
 Tuning tuning() { return g_tuning; }
 
@@ -696,6 +696,7 @@ struct SceneDev {
 struct Scenes {
   const SceneDev* s;
   int count;
+  int32_t* queue = nullptr;  // persistent launches: one work counter per XCD, zeroed before the launch
 };
 
 __device__ __forceinline__ int scene_of(const Scenes& S, int t) {
@@ -764,6 +765,11 @@ constexpr int kQuadBelowXsph = 36000;
 // streamed; 127k: 59 / 81 against 60 / 84).  Lambda's stream walk already
 // batches within rows and stays ahead (127k: 58 us streamed, 64 by row).
 constexpr int kRowWalk = 64;
+// Kernel template bit: the large-scene passes' other stream walk (the
+// autotuner's walk knob, FxTuning.walk = 2): lambda streams without row
+// batches, delta streams in row batches, XSPH streams with unconditional
+// loads.  Every walk visits the same candidates in the same order.
+constexpr int kWalkAlt = 512;
 constexpr int kRowBelow = 1 << 17;
 
 struct PV {
@@ -1046,6 +1052,26 @@ __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restr
 // the DPP reads need.
 constexpr int kQuadWalk = 128;
 
+// Persistent launches (kernel template bit kPersistent; the batched passes,
+// FxTuning.persistent): a grid of a few workgroups per CU, a multiple of the
+// chip's 8 XCDs, works through the launch's virtual blocks.  The workgroups
+// an XCD runs (blockIdx % 8) share that XCD's contiguous eighth of the
+// virtual blocks, so each XCD's L2 serves one stretch of sorted slots, and
+// take them in turn from the XCD's counter in S.queue (zeroed before the
+// launch): a workgroup that drew cheap blocks draws again.  A workgroup
+// leaves when its XCD's range is used up.
+constexpr int kPersistent = 256;
+__device__ __forceinline__ int persistent_take(int32_t* queue, long long nthreads) {
+  __shared__ int s_vb;
+  const int nb = (int)((nthreads + blockDim.x - 1) / blockDim.x);
+  const int x = (int)blockIdx.x % kXcds, per = (nb + kXcds - 1) / kXcds;
+  __syncthreads();  // the previous block's readers of s_vb are done
+  if (threadIdx.x == 0) s_vb = atomicAdd(&queue[x], 1);
+  __syncthreads();
+  const int j = s_vb, vb = x * per + j;
+  return (j < per && vb < nb) ? vb : -1;
+}
+
 // B: rounds of loads in flight per lane (dam break, 25,600 particles, B = 2 /
 // 4 / 8: lambda 29.9 / 25.3 / 23.9 us, delta 30.5 / 28.7 / 28.2, XSPH 43.0 /
 // 44.0 / 47.8; at 65k-90k particles B = 8 costs delta 2% and lambda up to 2%,
@@ -1222,16 +1248,13 @@ struct XsSums {
 // (dist2_exact, then lam_dens terms added in the reference's order).
 // V = MODE + 8 (batched scenes) + kRowWalk or kQuadWalk
 template <int V>
-__global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
-                                                float4* __restrict__ pred_s,
-                                                const int32_t* __restrict__ off_,
-                                                float2* __restrict__ dl_s,
-                                                const int32_t* __restrict__ sidx,
-                                                float4* __restrict__ lambda4, Lists L, Scenes S,
-                                                int iter = -1) {
+__device__ __forceinline__ void lambda_body(int gt_in, int n, Grid g_, Consts k_, float4* __restrict__ pred_s,
+                                            const int32_t* __restrict__ off_, float2* __restrict__ dl_s,
+                                            const int32_t* __restrict__ sidx, float4* __restrict__ lambda4, Lists L,
+                                            Scenes S, int iter) {
   constexpr int MODE = V & 3;
   constexpr bool QUAD = (V & kQuadWalk) != 0;
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gt = gt_in;
   const int t = QUAD ? gt >> 2 : gt;
   if (t >= n) return;
   Grid g = g_;
@@ -1263,7 +1286,7 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
       visit_quad<8>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const LamTerm& tt) { sum.add(tt); });
       if (gt & 3) return;
     } else {
-      visit_auto<MODE | ((V & kRowWalk) ? 0 : VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0))>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+      visit_auto<MODE | ((V & kRowWalk) ? 0 : (V & kWalkAlt) ? VISIT_STREAM : VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0))>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2_exact(dx, dy, dz);
         if (r2 < h2) {
@@ -1289,26 +1312,39 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
   if (lambda4) lambda4[min(max(sidx[t], 0), n - 1)] = make_float4(rho, lambda, 0.0f, 0.0f);
 }
 
+template <int V>
+__global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
+                                                float4* __restrict__ pred_s,
+                                                const int32_t* __restrict__ off_,
+                                                float2* __restrict__ dl_s,
+                                                const int32_t* __restrict__ sidx,
+                                                float4* __restrict__ lambda4, Lists L, Scenes S,
+                                                int iter = -1) {
+  if constexpr ((V & kPersistent) != 0) {
+    constexpr bool QUADW = (V & kQuadWalk) != 0;
+    const long long nth = (QUADW ? 4LL * n : (long long)n);
+    for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
+      lambda_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, g_, k_, pred_s, off_, dl_s, sidx, lambda4, L, S, iter);
+  } else {
+    lambda_body<V>(blockIdx.x * blockDim.x + threadIdx.x, n, g_, k_, pred_s, off_, dl_s, sidx, lambda4, L, S, iter);
+  }
+}
+
 // shaders/sph_delta.glsl:22-85 fused with shaders/sph_apply_delta.glsl:10-20
 // (Jacobi ping-pong: reads pred_in, writes pred_out) and, on the last
 // iteration, shaders/sph_velocity.glsl:12-24 into vel_s (w = own density).
 // delta4 (canonical, optional) receives the correction itself.  With lists,
 // each particle's accumulated |dp| feeds the global displacement bound.
 template <int V>
-__global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
-                                               const float4* __restrict__ pred_in,
-                                               const float2* __restrict__ dl_s,
-                                               const int32_t* __restrict__ off_,
-                                               float4* __restrict__ pred_out,
-                                               const float4* __restrict__ pos_s,
-                                               float4* __restrict__ vel_s,
-                                               const int32_t* __restrict__ sidx,
-                                               float4* __restrict__ delta4, Lists L,
-                                               const int32_t* __restrict__ skey, Scenes S,
-                                               int iter = -1) {
+__device__ __forceinline__ void delta_body(int gt_in, int n, Grid g_, Consts k_, const float4* __restrict__ pred_in,
+                                           const float2* __restrict__ dl_s, const int32_t* __restrict__ off_,
+                                           float4* __restrict__ pred_out, const float4* __restrict__ pos_s,
+                                           float4* __restrict__ vel_s, const int32_t* __restrict__ sidx,
+                                           float4* __restrict__ delta4, Lists L, const int32_t* __restrict__ skey,
+                                           Scenes S, int iter) {
   constexpr int MODE = V & 3;
   constexpr bool QUAD = (V & kQuadWalk) != 0;
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gt = gt_in;
   const int t = QUAD ? gt >> 2 : gt;
   if (t < n) {
     Grid g = g_;
@@ -1356,7 +1392,7 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
         visit_quad<4>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const DelTerm& tt) { sum.add(tt); });
         if (gt & 3) return;
       } else {
-        visit_auto<MODE | ((V & kRowWalk) ? 0 : VISIT_STREAM)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+        visit_auto<MODE | ((V & kRowWalk) ? 0 : (V & kWalkAlt) ? VISIT_STREAM | VISIT_ROW_BATCH : VISIT_STREAM)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
           const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
           const float r2 = dist2_exact(dx, dy, dz);
           if (r2 < h2 && r2 > 1e-12f && q != t) sum.add(DelTerm{del_coef(k, li, pj.w, inv_wq, scorr, r2), dx, dy, dz});
@@ -1385,6 +1421,28 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
   }
 }
 
+template <int V>
+__global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
+                                               const float4* __restrict__ pred_in,
+                                               const float2* __restrict__ dl_s,
+                                               const int32_t* __restrict__ off_,
+                                               float4* __restrict__ pred_out,
+                                               const float4* __restrict__ pos_s,
+                                               float4* __restrict__ vel_s,
+                                               const int32_t* __restrict__ sidx,
+                                               float4* __restrict__ delta4, Lists L,
+                                               const int32_t* __restrict__ skey, Scenes S,
+                                               int iter = -1) {
+  if constexpr ((V & kPersistent) != 0) {
+    constexpr bool QUADW = (V & kQuadWalk) != 0;
+    const long long nth = (QUADW ? 4LL * n : (long long)n);
+    for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
+      delta_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter);
+  } else {
+    delta_body<V>(blockIdx.x * blockDim.x + threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter);
+  }
+}
+
 // shaders/sph_velocity.glsl:12-24 in sorted space (used when iterations = 0).
 __global__ void k_velocity_s(int n, float dt, const float4* __restrict__ pred_s,
                              const float4* __restrict__ pos_s, const float2* __restrict__ dl_s,
@@ -1402,22 +1460,19 @@ __global__ void k_velocity_s(int n, float dt, const float4* __restrict__ pred_s,
 // (plus pos_s for the splat); otherwise write the correction to delta4 only.
 // V = 4 * FUSE + MODE (one template argument keeps launch macros simple)
 template <int V>
-__global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Consts k_,
-                                              Collider col_, const float4* __restrict__ pred_s,
-                                              const float4* __restrict__ vel_s,
-                                              const int32_t* __restrict__ off_,
-                                              const int32_t* __restrict__ sidx,
-                                              float4* __restrict__ delta4,
-                                              float4* __restrict__ pos4, float4* __restrict__ vel4,
-                                              float4* __restrict__ pos_s,
-                                              float4* __restrict__ pred4_out, Lists L, Scenes S,
-                                              float4* __restrict__ normal4 = nullptr, int normal_by_slot = 0) {
+__device__ __forceinline__ void xsph_body(int gt_in, int n, FxParams p_, Grid g_, Consts k_, Collider col_,
+                                          const float4* __restrict__ pred_s, const float4* __restrict__ vel_s,
+                                          const int32_t* __restrict__ off_, const int32_t* __restrict__ sidx,
+                                          float4* __restrict__ delta4, float4* __restrict__ pos4,
+                                          float4* __restrict__ vel4, float4* __restrict__ pos_s,
+                                          float4* __restrict__ pred4_out, Lists L, Scenes S,
+                                          float4* __restrict__ normal4, int normal_by_slot) {
   constexpr bool FUSE = (V & 4) != 0;
   constexpr bool SPLIT = (V & 16) != 0;  // dv in sorted order for k_finalize_s
   constexpr bool NORMAL = (V & 32) != 0;  // also the next substep's normal pass (k_normal)
   constexpr int MODE = V & 3;
   constexpr bool QUAD = (V & kQuadWalk) != 0;
-  const int gt = xcd_block() * blockDim.x + threadIdx.x;
+  const int gt = gt_in;
   const int t = QUAD ? gt >> 2 : gt;
   if (t >= n) return;
   FxParams p = p_;
@@ -1467,7 +1522,7 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
       };
       // xs_term and XsSums::add, written out so each sum is updated as soon
       // as its weight is known (fewer live registers; the same operations)
-      visit_auto<MODE | ((V & kRowWalk) ? 0 : VISIT_STREAM | VISIT_MASKED_LOADS)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
+      visit_auto<MODE | ((V & kRowWalk) ? 0 : (V & kWalkAlt) ? VISIT_STREAM : VISIT_STREAM | VISIT_MASKED_LOADS)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const PV& c) -> bool {
 #pragma clang fp contract(off)
         const float4 vj = c.v;
         const float dx = pi.x - c.p.x, dy = pi.y - c.p.y, dz = pi.z - c.p.z;
@@ -1524,6 +1579,27 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
   if (pos_s) pos_s[t] = pf;
 }
 
+template <int V>
+__global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Consts k_,
+                                              Collider col_, const float4* __restrict__ pred_s,
+                                              const float4* __restrict__ vel_s,
+                                              const int32_t* __restrict__ off_,
+                                              const int32_t* __restrict__ sidx,
+                                              float4* __restrict__ delta4,
+                                              float4* __restrict__ pos4, float4* __restrict__ vel4,
+                                              float4* __restrict__ pos_s,
+                                              float4* __restrict__ pred4_out, Lists L, Scenes S,
+                                              float4* __restrict__ normal4 = nullptr, int normal_by_slot = 0) {
+  if constexpr ((V & kPersistent) != 0) {
+    constexpr bool QUADW = (V & kQuadWalk) != 0;
+    const long long nth = (QUADW ? 4LL * n : (long long)n);
+    for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
+      xsph_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, p_, g_, k_, col_, pred_s, vel_s, off_, sidx, delta4, pos4, vel4, pos_s, pred4_out, L, S, normal4, normal_by_slot);
+  } else {
+    xsph_body<V>(xcd_block() * blockDim.x + threadIdx.x, n, p_, g_, k_, col_, pred_s, vel_s, off_, sidx, delta4, pos4, vel4, pos_s, pred4_out, L, S, normal4, normal_by_slot);
+  }
+}
+
 // shaders/sph_finalize.glsl:15-96 over sorted slots, after a SPLIT XSPH pass
 // left dv in sorted order: apply dv, collide, clamp, write the canonical
 // position / velocity (and correction / prediction when asked) and pos_s.
@@ -1540,16 +1616,13 @@ struct NormalMark {
 };
 
 template <int V>
-__global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p_, Collider col_,
-                                                    const float4* __restrict__ pred_s,
-                                                    const float4* __restrict__ vel_s,
-                                                    const float4* __restrict__ dv_s,
-                                                    const int32_t* __restrict__ sidx,
-                                                    float4* __restrict__ delta4, float4* __restrict__ pos4,
-                                                    float4* __restrict__ vel4, float4* __restrict__ pos_s,
-                                                    float4* __restrict__ pred4_out, Scenes S,
-                                                    NormalMark mk = NormalMark{}) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void finalize_s_body(int gt_in, int n, FxParams p_, Collider col_,
+                                                const float4* __restrict__ pred_s, const float4* __restrict__ vel_s,
+                                                const float4* __restrict__ dv_s, const int32_t* __restrict__ sidx,
+                                                float4* __restrict__ delta4, float4* __restrict__ pos4,
+                                                float4* __restrict__ vel4, float4* __restrict__ pos_s,
+                                                float4* __restrict__ pred4_out, Scenes S, NormalMark mk) {
+  const int t = gt_in;
   if (mk.flags && t == 0) {
     mk.flags[kHdrNormals] = mk.state;
     mk.flags[kHdrH] = mk.key.hbits;
@@ -1578,6 +1651,26 @@ __global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p_, Collider
   if (pos_s) pos_s[t] = pf;
 }
 
+template <int V>
+__global__ __launch_bounds__(256) void k_finalize_s(int n, FxParams p_, Collider col_,
+                                                    const float4* __restrict__ pred_s,
+                                                    const float4* __restrict__ vel_s,
+                                                    const float4* __restrict__ dv_s,
+                                                    const int32_t* __restrict__ sidx,
+                                                    float4* __restrict__ delta4, float4* __restrict__ pos4,
+                                                    float4* __restrict__ vel4, float4* __restrict__ pos_s,
+                                                    float4* __restrict__ pred4_out, Scenes S,
+                                                    NormalMark mk = NormalMark{}) {
+  if constexpr ((V & kPersistent) != 0) {
+    constexpr bool QUADW = (V & kQuadWalk) != 0;
+    const long long nth = (long long)n;
+    for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
+      finalize_s_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, p_, col_, pred_s, vel_s, dv_s, sidx, delta4, pos4, vel4, pos_s, pred4_out, S, mk);
+  } else {
+    finalize_s_body<V>(blockIdx.x * blockDim.x + threadIdx.x, n, p_, col_, pred_s, vel_s, dv_s, sidx, delta4, pos4, vel4, pos_s, pred4_out, S, mk);
+  }
+}
+
 // How a substep's normal pass decides on the device what to do (flags NULL:
 // always compute).  With no grid for (n, nc) in the workspace header it
 // writes zero normals and sets the header's error word instead of reading
@@ -1595,14 +1688,11 @@ struct NormalPick {
 // shaders/sph_normal.glsl:24-81 over the previous grid; pred_s.w = the
 // neighbour's density from the last lambda pass.  Writes canonical normal4.
 template <int V>
-__global__ __launch_bounds__(256) void k_normal(int n, Grid g_, Consts k_,
-                                                const float4* __restrict__ pred_s,
-                                                const int32_t* __restrict__ off_,
-                                                const int32_t* __restrict__ sidx,
-                                                float4* __restrict__ normal4, Lists L, Scenes S,
-                                                NormalPick pk = NormalPick{}) {
+__device__ __forceinline__ void normal_body(int gt_in, int n, Grid g_, Consts k_, const float4* __restrict__ pred_s,
+                                            const int32_t* __restrict__ off_, const int32_t* __restrict__ sidx,
+                                            float4* __restrict__ normal4, Lists L, Scenes S, NormalPick pk) {
   constexpr int MODE = V & 3;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = gt_in;
   if (pk.flags) {
     if (!header_grid_ok(pk.flags, n, pk.nc)) {
       if (t < n) normal4[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1658,6 +1748,23 @@ __global__ __launch_bounds__(256) void k_normal(int n, Grid g_, Consts k_,
   const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
   const float curvature = (mag > 1e-4f) ? -lap / mag : 0.0f;
   normal4[min(max(sidx[t], 0), n - 1)] = make_float4(nx, ny, nz, curvature);
+}
+
+template <int V>
+__global__ __launch_bounds__(256) void k_normal(int n, Grid g_, Consts k_,
+                                                const float4* __restrict__ pred_s,
+                                                const int32_t* __restrict__ off_,
+                                                const int32_t* __restrict__ sidx,
+                                                float4* __restrict__ normal4, Lists L, Scenes S,
+                                                NormalPick pk = NormalPick{}) {
+  if constexpr ((V & kPersistent) != 0) {
+    constexpr bool QUADW = (V & kQuadWalk) != 0;
+    const long long nth = (long long)n;
+    for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
+      normal_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, g_, k_, pred_s, off_, sidx, normal4, L, S, pk);
+  } else {
+    normal_body<V>(blockIdx.x * blockDim.x + threadIdx.x, n, g_, k_, pred_s, off_, sidx, normal4, L, S, pk);
+  }
 }
 
 
@@ -1812,14 +1919,50 @@ static int validate(const FxParams* p, const FxState* s, bool need_work) {
 // text)
 #define QLAUNCH(kernel, V, n, quad_below, row_below, st, ...)                                             \
   do {                                                                                                    \
-    if ((n) >= (quad_below) && (n) >= (row_below))                                                        \
-      NLAUNCH(kernel<V>, n, st, __VA_ARGS__);                                                             \
-    else if ((n) >= (quad_below))                                                                         \
+    if ((n) >= (quad_below) && (n) >= (row_below)) {                                                      \
+      if (g_tuning.walk == 1)                                                                             \
+        NLAUNCH(kernel<V + kRowWalk>, n, st, __VA_ARGS__);                                                \
+      else if (g_tuning.walk == 2)                                                                        \
+        NLAUNCH(kernel<V + kWalkAlt>, n, st, __VA_ARGS__);                                                \
+      else                                                                                                \
+        NLAUNCH(kernel<V>, n, st, __VA_ARGS__);                                                           \
+    } else if ((n) >= (quad_below))                                                                       \
       NLAUNCH(kernel<V + kRowWalk>, n, st, __VA_ARGS__);                                                  \
     else                                                                                                  \
       FX_LAUNCH(kernel<V + kQuadWalk>, dim3(grid_for(4LL * (n), g_tuning.block)), dim3(g_tuning.block), st, \
                 __VA_ARGS__);                                                                             \
   } while (0)
+
+// a batched pass: flat, or persistent with FxTuning.persistent workgroups per CU
+// (V unparenthesised, as above)
+#define BLAUNCH(kernel, V, n, st, ...)                                                                   \
+  do {                                                                                                   \
+    if (g_tuning.persistent > 0) {                                                                       \
+      FX_TRY(fill(S.queue, kXcds, 0, st));                                                               \
+      FX_LAUNCH(kernel<V + kPersistent>, dim3(persistent_grid(n, g_tuning.block)), dim3(g_tuning.block),   \
+                st, __VA_ARGS__);                                                                        \
+    } else                                                                                               \
+      NLAUNCH(kernel<V>, n, st, __VA_ARGS__);                                                            \
+  } while (0)
+
+// Workgroups of a persistent launch over n threads: g_tuning.persistent per
+// CU, a multiple of the 8 XCDs, no more than the flat launch would use.
+static int g_cus = 0;
+static unsigned persistent_grid(long long n, int block) {
+  if (g_cus <= 0) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+      g_cus = cus;
+    else
+      g_cus = 256;
+  }
+  long long g = (long long)g_cus * g_tuning.persistent;
+  const long long flat = (n + block - 1) / block;
+  if (g > flat) g = flat;
+  g = (g + kXcds - 1) / kXcds * kXcds;
+  return (unsigned)(g < kXcds ? kXcds : g);
+}
 
 static Lists make_lists(const Ws& w, const Consts& k, bool enabled) {
   Lists L;
@@ -2285,7 +2428,8 @@ static int batch_step(const FxScene* scenes, int count, float* pos, float* vel, 
   float4* pred = (float4*)(base + b.pred_off);
   void* wsp = base + b.solver_off;
   Ws w = bind_ws(workspace_layout(b.virt), wsp);
-  Scenes S{table, count};
+  // persistent launches draw their blocks from eight counters in the flags block
+  Scenes S{table, count, w.flags + 32};
   const int n = (int)b.ntot, nc = (int)b.ctot;
   if (load) {
     // the per-scene table lives in the workspace so captured substeps read it
@@ -2300,7 +2444,7 @@ static int batch_step(const FxScene* scenes, int count, float* pos, float* vel, 
     FX_TRY(require_grid(workspace_layout(b.virt), wsp, n, nc, st, "batch: surface tension"));
     // the pick only guards the grid: the batch never fuses normals into XSPH
     const NormalPick pk{w.flags, nullptr, NormalKey{}, nc};
-    NLAUNCH(k_normal<8 + VISIT_FULL>, n, st, n, Grid{}, Consts{}, w.pred_s[0], w.off, w.sidx, normal, L0, S, pk);
+    BLAUNCH(k_normal, 8 + VISIT_FULL, n, st, n, Grid{}, Consts{}, w.pred_s[0], w.off, w.sidx, normal, L0, S, pk);
   }
   const int iters = b.iterations;
   const int cur0 = load ? 0 : (iters & 1);
@@ -2317,10 +2461,10 @@ static int batch_step(const FxScene* scenes, int count, float* pos, float* vel, 
   if (load || n == 0) return check_launch("batch");
   int cur = cur0;
   for (int it = 0; it < iters; ++it) {
-    NLAUNCH(k_lambda<8 + VISIT_FULL>, n, st, n, Grid{}, Consts{}, w.pred_s[cur], w.off, w.dl_s, w.sidx,
+    BLAUNCH(k_lambda, 8 + VISIT_FULL, n, st, n, Grid{}, Consts{}, w.pred_s[cur], w.off, w.dl_s, w.sidx,
             (float4*)nullptr, L0, S, it);
     // each scene writes its velocity on its own last iteration
-    NLAUNCH(k_delta<8 + VISIT_FULL>, n, st, n, Grid{}, Consts{}, (const float4*)w.pred_s[cur],
+    BLAUNCH(k_delta, 8 + VISIT_FULL, n, st, n, Grid{}, Consts{}, (const float4*)w.pred_s[cur],
             (const float2*)w.dl_s, w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s, w.vel_s, w.sidx,
             (float4*)nullptr, L0, (const int32_t*)nullptr, S, it);
     cur ^= 1;
@@ -2335,7 +2479,7 @@ static int batch_step(const FxScene* scenes, int count, float* pos, float* vel, 
   // XSPH leaves dv by slot (in pred_s[1], free after the iterations) and the
   // finalize runs on its own, as in the plain substep: without the collider
   // search the neighbour pass holds fewer registers (91 -> 73 VGPRs)
-  NLAUNCH(k_xsph<8 + 16 + VISIT_FULL>, n, st, n, FxParams{}, Grid{}, Consts{}, Collider{}, (const float4*)w.pred_s[0],
+  BLAUNCH(k_xsph, 8 + 16 + VISIT_FULL, n, st, n, FxParams{}, Grid{}, Consts{}, Collider{}, (const float4*)w.pred_s[0],
           (const float4*)w.vel_s, w.off, w.sidx, w.pred_s[1], (float4*)nullptr, (float4*)nullptr, (float4*)nullptr,
           (float4*)nullptr, L0, S);
   LAUNCH(k_finalize_s<8>, n, st, n, FxParams{}, Collider{}, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
@@ -2359,8 +2503,13 @@ int fx_set_tuning(const FxTuning* t) {
     return fail(FX_EINVAL, "fx_set_tuning: list_skin %g outside [0, 1]", (double)t->list_skin);
   if (t->block != 64 && t->block != 128 && t->block != 256)
     return fail(FX_EINVAL, "fx_set_tuning: block %d not in {64, 128, 256}", t->block);
+  if (t->persistent < 0 || t->persistent > 32)
+    return fail(FX_EINVAL, "fx_set_tuning: persistent %d not in [0, 32]", t->persistent);
+  if (t->walk < 0 || t->walk > 2) return fail(FX_EINVAL, "fx_set_tuning: walk %d not in {0, 1, 2}", t->walk);
   g_tuning.skin = t->list_skin;
   g_tuning.block = t->block;
+  g_tuning.persistent = t->persistent;
+  g_tuning.walk = t->walk;
   return FX_OK;
 }
 
@@ -2368,6 +2517,8 @@ int fx_get_tuning(FxTuning* t) {
   if (!t) return fail(FX_EINVAL, "fx_get_tuning: null");
   t->list_skin = g_tuning.skin;
   t->block = g_tuning.block;
+  t->persistent = g_tuning.persistent;
+  t->walk = g_tuning.walk;
   return FX_OK;
 }
 

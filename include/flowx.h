@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define FX_ABI_VERSION 1
+#define FX_ABI_VERSION 2
 
 /* error codes */
 #define FX_OK 0
@@ -151,11 +151,17 @@ int fx_profile(int enable);
 
 /* Launch tuning of the neighbour passes (no reference counterpart), set by
  * the autotuner.  list_skin: neighbour-list margin in units of h (0 = always
- * scan the full 27-cell stencil); block: 64, 128 or 256 threads.  Results do
- * not depend on either. */
+ * scan the full 27-cell stencil); block: 64, 128 or 256 threads; walk: how a
+ * large scene's neighbour passes stage the stencil's cell rows into batches
+ * of loads in flight (0: per pass default, 1: row by row, 2: the other stream
+ * walk); persistent: the batched passes' launch, 0 = one workgroup per block
+ * of particles, k > 0 = k workgroups per CU looping over them (XCD-contiguous
+ * ranges).  Results do not depend on any of them. */
 typedef struct FxTuning {
   float list_skin;
   int32_t block;
+  int32_t persistent;
+  int32_t walk;
 } FxTuning;
 int fx_set_tuning(const FxTuning* t);
 int fx_get_tuning(FxTuning* t);

@@ -21,7 +21,7 @@ def test_library_exports_every_declared_symbol(fx):
     lib = fx._lib.lib()
     missing = [name for name in _declared() if not hasattr(lib, name)]
     assert not missing, f"declared in flowx.h but not exported: {missing}"
-    assert lib.fx_abi_version() == 1
+    assert lib.fx_abi_version() == 2
 
 
 def test_every_declared_symbol_has_a_ctypes_signature(fx):
@@ -88,7 +88,7 @@ def test_autotune_file_is_applied(fx, tmp_path):
         assert t["block"] == 128 and abs(t["list_skin"] - 0.2) < 1e-7
         assert autotune.apply(tmp_path / "missing.json") is None
     finally:
-        _lib.set_tuning(list_skin=saved["list_skin"], block=saved["block"])
+        _lib.set_tuning(**saved)
 
 
 def test_tuning_round_trip_and_rejects_bad_values(fx):
@@ -101,7 +101,10 @@ def test_tuning_round_trip_and_rejects_bad_values(fx):
         _lib.set_tuning(list_skin=0.1, block=256)
         t = _lib.get_tuning()
         assert t["block"] == 256 and abs(t["list_skin"] - 0.1) < 1e-7
-        for bad in ({"block": 96}, {"list_skin": -0.5}, {"list_skin": 2.0}):
+        _lib.set_tuning(persistent=4, walk=2)
+        t = _lib.get_tuning()
+        assert t["persistent"] == 4 and t["walk"] == 2
+        for bad in ({"block": 96}, {"list_skin": -0.5}, {"list_skin": 2.0}, {"persistent": -1}, {"walk": 3}):
             with pytest.raises(RuntimeError):
                 _lib.set_tuning(**bad)
             assert _lib.get_tuning() == t

@@ -164,17 +164,24 @@ def test_batch_graph_replay_matches_eager(fx):
 
 
 def test_autotune_sweep_keeps_results_identical(fx, tmp_path):
-    """The autotuner's sweep (block size x list skin) on a small scene: every
-    setting reproduces the first setting's positions, and the fastest is
-    written and applied."""
+    """The autotuner's sweep on a 204,800-particle scene (block size x list
+    skin x stencil walk, where every full-stencil pass streams) and on a
+    three-scene batch (flat and persistent launches): every setting
+    reproduces the first setting's positions, and the fastest is written and
+    applied."""
     from flowx_amd import _lib, autotune
+    from flowx_amd.config import converging_update
 
     saved = _lib.get_tuning()
     try:
-        cfg = fx.dam_break_config()
-        out = autotune.tune(fx, cfg, blocks=(128, 256), skins=(0.0, 0.2), warmup=2, steps=3,
-                            path=tmp_path / "t.json")
-        assert len(out["results"]) == 4 and all(r["same_result"] for r in out["results"])
+        cfg = converging_update(fx.lattice_config((-1.0, -1.0, 0.0), (1.0, 1.0, 2.0), 0.025, (80, 80, 32),
+                                                  iterations=4, viscosity=0.01, surface_tension=0.001))
+        cfg.extra.update(colliders=[("box", (-0.2, -0.2, 0.0), (0.2, 0.2, 0.3))], collider_resolution=40)
+        out = autotune.tune(fx, cfg, blocks=(64, 256), skins=(0.0, 0.2), walks=(0, 1, 2), batch_cfgs=_scenes(fx),
+                            persistents=(0, 2, 8), warmup=1, steps=2, path=tmp_path / "t.json")
+        assert len(out["results"]) == 8 and all(r["same_result"] for r in out["results"])
+        assert len(out["batch_results"]) == 3 and all(r["same_result"] for r in out["batch_results"])
         assert autotune.apply(tmp_path / "t.json") == out["best"]
+        assert _lib.get_tuning()["walk"] == out["best"]["walk"]
     finally:
-        _lib.set_tuning(list_skin=saved["list_skin"], block=saved["block"])
+        _lib.set_tuning(**saved)
