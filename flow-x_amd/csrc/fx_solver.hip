@@ -1033,6 +1033,192 @@ __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restr
   }
 }
 
+// LDS-staged stencil rows (kernel template bit kLdsWalk; lambda and delta on
+// large scenes).  The workgroup's kTileBlock consecutive sorted slots were
+// binned into build cells kf..kl (skey of its first and last slot).  For each
+// stencil row (dz, dy), with d its key offset dz*cy*cx + dy*cx, the cells
+// kf + d - 1 .. kl + d + 1 hold one contiguous run of slots: the workgroup
+// copies those nine runs (the window of row r) into LDS once, with coalesced
+// loads, and each lane then reads its candidates from LDS.  A lane whose
+// stencil cell is still a build cell of the block finds each of its row runs
+// inside the window of the same row; one that moved to a neighbouring
+// row of cells since the build (its cell minus one row offset lies in kf..kl)
+// finds them in the windows shifted by that offset, where they exist.  Any
+// run that is not inside a staged window - the lane moved further, the row
+// falls outside the shifted windows, or the windows overflowed the tile - is
+// read from global memory instead.  Candidates reach f in the same order
+// whichever memory serves them.
+// Measured on the 1.64M scene (FxTuning.walk = 3; lambda us, the stream walk
+// 334): 425 with 256-particle workgroups and 2,496 tile slots (40 KB, four
+// workgroups per CU), 505 / 592 / 703 with 3,200 / 1,920 / 1,152 slots for
+// 256 / 128 / 64 particles (three, five, eight workgroups per CU).  The time
+// follows the waves per CU (16, 12, 10, 8 against the stream walk's 32): the
+// walk is bound by each wave's chain of dependent instructions, not by the
+// memory pipeline it unloads (TA busy 84 -> 46%, TD 89 -> 68%).  Two more
+// limits: a 256-slot workgroup's nine windows hold 2,483 slots at the median
+// and 3,184 at p99, and by the last Jacobi iteration 30% of the particles sit
+// in another (z, y) row of cells than their build cell (42% in another
+// cell), so 229 of the stream walk's 291 loads per wave remain.  On h/2
+// lattices of 204,800 / 819,200 / 1,638,400 particles lambda takes 79 / 223
+// / 425 us against the stream walk's 68 / 172 / 334 and delta 70 / 203 /
+// 389 against 69 / 168 / 324.  Kept as a walk the autotuner measures
+// (bit-identical to the others); it has not won on a scene measured.
+constexpr int kLdsWalk = 1024;
+constexpr int kTileBlock = 256;
+constexpr int kTileCap = 2496;  // float4 slots: four workgroups share a CU's 160 KB
+constexpr int kTileGlobal = std::numeric_limits<int>::min();
+
+struct Tile {
+  float4 c[kTileCap + 8];  // a batch may read up to 7 slots past a window
+  int a[9], len[9], base[9];
+  int kf, kl;
+};
+
+// Fill T for the workgroup whose first slot is t0.  Every thread of the
+// workgroup calls it (it synchronises).
+__device__ __forceinline__ void stage_tile(Tile& T, const Grid& g, const int32_t* __restrict__ off,
+                                           const int32_t* __restrict__ skey, const float4* __restrict__ src,
+                                           int t0, int n) {
+  const int tid = threadIdx.x;
+  if (tid < 9) {
+    const int t1 = min(t0 + kTileBlock, n) - 1;
+    const int kf = skey[t0], kl = skey[t1];
+    const long long nc = (long long)g.cx * g.cy * g.cz;
+    const long long d = (long long)(tid / 3 - 1) * g.cy * g.cx + (long long)(tid % 3 - 1) * g.cx;
+    const long long lo = max((long long)kf + d - 1, 0LL), hi = min((long long)kl + d + 1, nc - 1);
+    int a = 0, len = 0;
+    if (lo <= hi) {
+      a = off[lo];
+      len = max(off[hi + 1] - a, 0);
+    }
+    T.a[tid] = a;
+    T.len[tid] = len;
+    if (tid == 0) {
+      T.kf = kf;
+      T.kl = kl;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int b = 0;
+    for (int r = 0; r < 9; ++r) {
+      const int l = min(T.len[r], kTileCap - b);
+      T.base[r] = b;
+      T.len[r] = l;
+      b += l;
+    }
+  }
+  __syncthreads();
+  const int total = T.base[8] + T.len[8];
+  int wa[9], wb[9];
+#pragma unroll
+  for (int w = 0; w < 9; ++w) {
+    wa[w] = T.a[w];
+    wb[w] = T.base[w];
+  }
+  constexpr int J = (kTileCap + kTileBlock - 1) / kTileBlock;
+  float4 v[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int e = tid + j * kTileBlock;
+    int r = 0;
+#pragma unroll
+    for (int w = 1; w < 9; ++w)
+      if (e >= wb[w]) r = w;  // the last window starting at or before e (empty ones are skipped)
+    // past the staged total the load is clamped to a valid slot and not stored
+    v[j] = src[min(max(wa[r] + (e - wb[r]), 0), n - 1)];
+  }
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int e = tid + j * kTileBlock;
+    if (e < total) T.c[e] = v[j];
+  }
+  __syncthreads();
+}
+
+// The full stencil of one particle through the staged tile, batches of B
+// inside each row.  ld(q) loads slot q from global memory (same type as the
+// tile's entries); f as for visit.
+template <int B, class LD, class F>
+__device__ __forceinline__ void visit_tile(const Grid& g, const int32_t* __restrict__ off, int max_scan, int bx,
+                                           int by, int bz, const Tile& T, LD&& ld, F&& f) {
+  const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
+  const int c = cell_index(g, bx, by, bz);
+  const int kf = T.kf, kl = T.kl;
+  // the row shift (sz, sy) with c - shift inside kf..kl; sz = 2: none
+  int sz = 2, sy = 0;
+  if (c >= kf && c <= kl) {
+    sz = 0;
+  } else {
+    const int cxy = g.cx * g.cy;
+#pragma unroll
+    for (int s = 0; s < 9; ++s) {
+      const int dz = s / 3 - 1, dy = s % 3 - 1;
+      const int ck = c - dz * cxy - dy * g.cx;
+      if (sz == 2 && ck >= kf && ck <= kl) {
+        sz = dz;
+        sy = dy;
+      }
+    }
+  }
+  int S[9], E[9], D[9];
+  bool big = false;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
+    const bool in = z >= 0 && z < g.cz && y >= 0 && y < g.cy;
+    const int c0 = in ? (z * g.cy + y) * g.cx + x0 : 0;
+    S[r] = in ? off[c0] : 0;
+    E[r] = in ? off[c0 + (x1 - x0) + 1] : 0;
+    big |= E[r] - S[r] > max_scan;
+    D[r] = kTileGlobal;
+    const int wz = r / 3 - 1 + sz, wy = r % 3 - 1 + sy;
+    if (wz >= -1 && wz <= 1 && wy >= -1 && wy <= 1) {
+      const int w = (wz + 1) * 3 + (wy + 1);
+      const int a = T.a[w];
+      if (S[r] >= a && E[r] <= a + T.len[w]) D[r] = T.base[w] - a;
+    }
+  }
+  if (big) {
+    // a row past max_scan: the per-cell cut of the row-by-row walk
+    visit<VISIT_FULL>(g, off, max_scan, bx, by, bz, 0, 0, Lists{}, ld, f);
+    return;
+  }
+  int k = S[0], e = E[0], dd = D[0], left = 8;
+  auto next_row = [&]() {
+    while (k >= e && left > 0) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        S[r] = S[r + 1];
+        E[r] = E[r + 1];
+        D[r] = D[r + 1];
+      }
+      k = S[0];
+      e = E[0];
+      dd = D[0];
+      --left;
+    }
+  };
+  next_row();
+#pragma unroll 1
+  while (k < e) {
+    decltype(ld(0)) d[B];
+    if (dd != kTileGlobal) {
+      const float4* tp = T.c + (k + dd);
+#pragma unroll
+      for (int u = 0; u < B; ++u) d[u] = tp[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < B; ++u) d[u] = ld(min(k + u, e - 1));
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (k + u < e) f(k + u, d[u]);
+    k += B;
+    next_row();
+  }
+}
+
 // Small scenes, full stencil, four lanes per particle (kernel template bit
 // kQuadWalk).  In a small scene most SIMDs hold one wave or none, and a pass
 // lasts as long as one wave's instruction stream; four lanes per particle
@@ -1251,7 +1437,7 @@ template <int V>
 __device__ __forceinline__ void lambda_body(int gt_in, int n, Grid g_, Consts k_, float4* __restrict__ pred_s,
                                             const int32_t* __restrict__ off_, float2* __restrict__ dl_s,
                                             const int32_t* __restrict__ sidx, float4* __restrict__ lambda4, Lists L,
-                                            Scenes S, int iter) {
+                                            Scenes S, int iter, const Tile* tile = nullptr) {
   constexpr int MODE = V & 3;
   constexpr bool QUAD = (V & kQuadWalk) != 0;
   const int gt = gt_in;
@@ -1286,7 +1472,7 @@ __device__ __forceinline__ void lambda_body(int gt_in, int n, Grid g_, Consts k_
       visit_quad<8>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const LamTerm& tt) { sum.add(tt); });
       if (gt & 3) return;
     } else {
-      visit_auto<MODE | ((V & kRowWalk) ? 0 : (V & kWalkAlt) ? VISIT_STREAM : VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0))>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+      auto body = [&](int q, const float4& pj) -> bool {
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2_exact(dx, dy, dz);
         if (r2 < h2) {
@@ -1294,7 +1480,11 @@ __device__ __forceinline__ void lambda_body(int gt_in, int n, Grid g_, Consts k_
           if (q != t && r2 > 1e-12f) sum.add_grad(lam_grad(0.0f, lam_coef(k, r2), dx, dy, dz));
         }
         return r2 < r2l;
-      });
+      };
+      if constexpr ((V & kLdsWalk) != 0)
+        visit_tile<6>(g, off, k.max_scan, bx, by, bz, *tile, ld, body);
+      else
+        visit_auto<MODE | ((V & kRowWalk) ? 0 : (V & kWalkAlt) ? VISIT_STREAM : VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0))>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
     }
   }
   float dens = sum.dens, gx = sum.gx, gy = sum.gy, gz = sum.gz, gn2 = sum.gn2;
@@ -1319,12 +1509,18 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
                                                 float2* __restrict__ dl_s,
                                                 const int32_t* __restrict__ sidx,
                                                 float4* __restrict__ lambda4, Lists L, Scenes S,
-                                                int iter = -1) {
+                                                int iter = -1, const int32_t* __restrict__ tkey = nullptr) {
   if constexpr ((V & kPersistent) != 0) {
     constexpr bool QUADW = (V & kQuadWalk) != 0;
     const long long nth = (QUADW ? 4LL * n : (long long)n);
     for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
       lambda_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, g_, k_, pred_s, off_, dl_s, sidx, lambda4, L, S, iter);
+  } else if constexpr ((V & kLdsWalk) != 0) {
+    // launched with kTileBlock threads; tkey: the sorted build keys
+    __shared__ Tile tile;
+    const int t0 = xcd_block() * kTileBlock;
+    stage_tile(tile, g_, off_, tkey, pred_s, t0, n);
+    lambda_body<V>(t0 + (int)threadIdx.x, n, g_, k_, pred_s, off_, dl_s, sidx, lambda4, L, S, iter, &tile);
   } else {
     lambda_body<V>(blockIdx.x * blockDim.x + threadIdx.x, n, g_, k_, pred_s, off_, dl_s, sidx, lambda4, L, S, iter);
   }
@@ -1341,7 +1537,7 @@ __device__ __forceinline__ void delta_body(int gt_in, int n, Grid g_, Consts k_,
                                            float4* __restrict__ pred_out, const float4* __restrict__ pos_s,
                                            float4* __restrict__ vel_s, const int32_t* __restrict__ sidx,
                                            float4* __restrict__ delta4, Lists L, const int32_t* __restrict__ skey,
-                                           Scenes S, int iter) {
+                                           Scenes S, int iter, const Tile* tile = nullptr) {
   constexpr int MODE = V & 3;
   constexpr bool QUAD = (V & kQuadWalk) != 0;
   const int gt = gt_in;
@@ -1392,12 +1588,16 @@ __device__ __forceinline__ void delta_body(int gt_in, int n, Grid g_, Consts k_,
         visit_quad<4>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const DelTerm& tt) { sum.add(tt); });
         if (gt & 3) return;
       } else {
-        visit_auto<MODE | ((V & kRowWalk) ? 0 : (V & kWalkAlt) ? VISIT_STREAM | VISIT_ROW_BATCH : VISIT_STREAM)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, [&](int q, const float4& pj) -> bool {
+        auto body = [&](int q, const float4& pj) -> bool {
           const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
           const float r2 = dist2_exact(dx, dy, dz);
           if (r2 < h2 && r2 > 1e-12f && q != t) sum.add(DelTerm{del_coef(k, li, pj.w, inv_wq, scorr, r2), dx, dy, dz});
           return false;
-        });
+        };
+        if constexpr ((V & kLdsWalk) != 0)
+          visit_tile<6>(g, off, k.max_scan, bx, by, bz, *tile, ld, body);
+        else
+          visit_auto<MODE | ((V & kRowWalk) ? 0 : (V & kWalkAlt) ? VISIT_STREAM | VISIT_ROW_BATCH : VISIT_STREAM)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
       }
     }
     const float ox = sum.ox, oy = sum.oy, oz = sum.oz;
@@ -1432,12 +1632,18 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
                                                const int32_t* __restrict__ sidx,
                                                float4* __restrict__ delta4, Lists L,
                                                const int32_t* __restrict__ skey, Scenes S,
-                                               int iter = -1) {
+                                               int iter = -1, const int32_t* __restrict__ tkey = nullptr) {
   if constexpr ((V & kPersistent) != 0) {
     constexpr bool QUADW = (V & kQuadWalk) != 0;
     const long long nth = (QUADW ? 4LL * n : (long long)n);
     for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
       delta_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter);
+  } else if constexpr ((V & kLdsWalk) != 0) {
+    // launched with kTileBlock threads; tkey: the sorted build keys
+    __shared__ Tile tile;
+    const int t0 = xcd_block() * kTileBlock;
+    stage_tile(tile, g_, off_, tkey, pred_in, t0, n);
+    delta_body<V>(t0 + (int)threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter, &tile);
   } else {
     delta_body<V>(blockIdx.x * blockDim.x + threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter);
   }
@@ -2136,7 +2342,10 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
   for (int it = 0; it < iters; ++it) {
     const bool last = it == iters - 1;
     float4* lam_out = (last && last_in_call && s.lambda) ? (float4*)s.lambda : (float4*)nullptr;
-    if (!lists)
+    if (!lists && g_tuning.walk == 3 && n >= kRowBelow)
+      FX_LAUNCH(k_lambda<VISIT_FULL + kLdsWalk>, dim3(grid_for(n, kTileBlock)), dim3(kTileBlock), st, n, g, k,
+                w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0}, -1, (const int32_t*)w.skey);
+    else if (!lists)
       QLAUNCH(k_lambda, VISIT_FULL, n, kQuadBelowLambda, 0, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else if (it == 0)
       NLAUNCH(k_lambda<VISIT_BUILD>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
@@ -2146,6 +2355,11 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
       NLAUNCH(k_delta<VISIT_LIST>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
               w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
               (float4*)nullptr, L, (const int32_t*)w.skey, Scenes{nullptr, 0});
+    else if (g_tuning.walk == 3 && n >= kRowBelow)
+      FX_LAUNCH(k_delta<VISIT_FULL + kLdsWalk>, dim3(grid_for(n, kTileBlock)), dim3(kTileBlock), st, n, g, k,
+                (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s,
+                last ? w.vel_s : (float4*)nullptr, w.sidx, (float4*)nullptr, L, (const int32_t*)nullptr,
+                Scenes{nullptr, 0}, -1, (const int32_t*)w.skey);
     else
       QLAUNCH(k_delta, VISIT_FULL, n, kQuadBelowDelta, kRowBelow, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
               w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
@@ -2505,7 +2719,7 @@ int fx_set_tuning(const FxTuning* t) {
     return fail(FX_EINVAL, "fx_set_tuning: block %d not in {64, 128, 256}", t->block);
   if (t->persistent < 0 || t->persistent > 32)
     return fail(FX_EINVAL, "fx_set_tuning: persistent %d not in [0, 32]", t->persistent);
-  if (t->walk < 0 || t->walk > 2) return fail(FX_EINVAL, "fx_set_tuning: walk %d not in {0, 1, 2}", t->walk);
+  if (t->walk < 0 || t->walk > 3) return fail(FX_EINVAL, "fx_set_tuning: walk %d not in {0, 1, 2, 3}", t->walk);
   g_tuning.skin = t->list_skin;
   g_tuning.block = t->block;
   g_tuning.persistent = t->persistent;

@@ -101,10 +101,11 @@ def test_tuning_round_trip_and_rejects_bad_values(fx):
         _lib.set_tuning(list_skin=0.1, block=256)
         t = _lib.get_tuning()
         assert t["block"] == 256 and abs(t["list_skin"] - 0.1) < 1e-7
-        _lib.set_tuning(persistent=4, walk=2)
+        _lib.set_tuning(persistent=4, walk=3)
         t = _lib.get_tuning()
-        assert t["persistent"] == 4 and t["walk"] == 2
-        for bad in ({"block": 96}, {"list_skin": -0.5}, {"list_skin": 2.0}, {"persistent": -1}, {"walk": 3}):
+        assert t["persistent"] == 4 and t["walk"] == 3
+        for bad in ({"block": 96}, {"list_skin": -0.5}, {"list_skin": 2.0}, {"persistent": -1}, {"walk": 4},
+                    {"walk": -1}):
             with pytest.raises(RuntimeError):
                 _lib.set_tuning(**bad)
             assert _lib.get_tuning() == t

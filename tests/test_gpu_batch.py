@@ -112,6 +112,47 @@ def test_walks_by_scene_size_are_bit_identical(fx, lattice):
     assert np.array_equal(batch.scene_velocities(0), s.read_velocities())
 
 
+@pytest.mark.parametrize("squeeze", [1.0, 0.5])
+def test_lds_tile_walk_is_bit_identical(fx, squeeze):
+    """Walk 3 (lambda and delta read their candidates from LDS tiles of the
+    workgroup's stencil rows, global memory for runs outside the tile) against
+    the default stream walks, bit for bit, on a 204,800-particle lattice with
+    a collider.  As seeded, particles leave their build rows of cells between
+    Jacobi iterations (runs found in shifted windows or read from global
+    memory); squeezed to half its height (16 particles per cell) the nine
+    windows of a workgroup overflow the tile and its later rows come from
+    global memory."""
+    import torch
+
+    from flowx_amd import _lib
+    from flowx_amd.config import converging_update
+
+    cfg = converging_update(fx.lattice_config((-1.0, -1.0, 0.0), (1.0, 1.0, 2.0), 0.025, (80, 80, 32), iterations=4,
+                                              viscosity=0.01, surface_tension=0.001))
+    cfg.extra.update(colliders=[("box", (-0.2, -0.2, 0.0), (0.2, 0.2, 0.3))], collider_resolution=40)
+    _, dt = cfg.substep_dt(1.0 / 24.0)
+    seed = fx.seed_positions(cfg)
+    seed[:, 2] = np.float32(cfg.lo[2]) + (seed[:, 2] - np.float32(cfg.lo[2])) * np.float32(squeeze)
+    saved = _lib.get_tuning()
+    out = []
+    try:
+        for walk in (0, 3):
+            _lib.set_tuning(walk=walk)
+            s = fx.Solver(cfg, keep_scratch=True)
+            s.reset()
+            s.load_state(seed, None)
+            s.substeps_n(dt, 3)
+            s.substeps_n(dt, 2)
+            torch.cuda.synchronize()
+            out.append([getattr(s, f).cpu().numpy() for f in ("pos", "vel", "lam", "pred")])
+            del s
+    finally:
+        _lib.set_tuning(**saved)
+    assert np.isfinite(out[0][0]).all()
+    for f, a, b in zip(("pos", "vel", "lam", "pred"), *out):
+        assert np.array_equal(a, b), f
+
+
 def test_batch_from_generated_scene_set_runs(fx):
     import torch
 
@@ -165,8 +206,8 @@ def test_batch_graph_replay_matches_eager(fx):
 
 def test_autotune_sweep_keeps_results_identical(fx, tmp_path):
     """The autotuner's sweep on a 204,800-particle scene (block size x list
-    skin x stencil walk, where every full-stencil pass streams) and on a
-    three-scene batch (flat and persistent launches): every setting
+    skin x stencil walk, where every full-stencil pass streams or reads LDS
+    tiles) and on a three-scene batch (flat and persistent launches): every setting
     reproduces the first setting's positions, and the fastest is written and
     applied."""
     from flowx_amd import _lib, autotune
@@ -177,9 +218,9 @@ def test_autotune_sweep_keeps_results_identical(fx, tmp_path):
         cfg = converging_update(fx.lattice_config((-1.0, -1.0, 0.0), (1.0, 1.0, 2.0), 0.025, (80, 80, 32),
                                                   iterations=4, viscosity=0.01, surface_tension=0.001))
         cfg.extra.update(colliders=[("box", (-0.2, -0.2, 0.0), (0.2, 0.2, 0.3))], collider_resolution=40)
-        out = autotune.tune(fx, cfg, blocks=(64, 256), skins=(0.0, 0.2), walks=(0, 1, 2), batch_cfgs=_scenes(fx),
+        out = autotune.tune(fx, cfg, blocks=(64, 256), skins=(0.0, 0.2), walks=(0, 1, 2, 3), batch_cfgs=_scenes(fx),
                             persistents=(0, 2, 8), warmup=1, steps=2, path=tmp_path / "t.json")
-        assert len(out["results"]) == 8 and all(r["same_result"] for r in out["results"])
+        assert len(out["results"]) == 10 and all(r["same_result"] for r in out["results"])
         assert len(out["batch_results"]) == 3 and all(r["same_result"] for r in out["batch_results"])
         assert autotune.apply(tmp_path / "t.json") == out["best"]
         assert _lib.get_tuning()["walk"] == out["best"]["walk"]
