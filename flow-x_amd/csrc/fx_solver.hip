@@ -1418,7 +1418,9 @@ __device__ __forceinline__ XsTerm xs_term(const Consts& k, const float4& vi, con
     o.dx = dx;
     o.dy = dy;
     o.dz = dz;
-    o.hr = __builtin_fmaf(-r2, __builtin_amdgcn_rsqf(r2), k.h);  // h - r
+    // h - r; a distinct particle at the same position (r2 = +0) gives h, as
+    // the reference's w_visc_lap(sqrt(r2)) does (r2 * rsq(r2) would be NaN)
+    o.hr = r2 > 0.0f ? __builtin_fmaf(-r2, __builtin_amdgcn_rsqf(r2), k.h) : k.h;
   }
   return o;
 }
@@ -1759,7 +1761,8 @@ __device__ __forceinline__ void xsph_body(int gt_in, int n, FxParams p_, Grid g_
             sum.nx = __builtin_fmaf(dx, gsc, sum.nx);
             sum.ny = __builtin_fmaf(dy, gsc, sum.ny);
             sum.nz = __builtin_fmaf(dz, gsc, sum.nz);
-            sum.lap = __builtin_fmaf(__builtin_fmaf(-r2, __builtin_amdgcn_rsqf(r2), k.h), irho, sum.lap);
+            const float hr = r2 > 0.0f ? __builtin_fmaf(-r2, __builtin_amdgcn_rsqf(r2), k.h) : k.h;  // as xs_term
+            sum.lap = __builtin_fmaf(hr, irho, sum.lap);
           }
         }
         return false;
@@ -1954,8 +1957,11 @@ __device__ __forceinline__ void normal_body(int gt_in, int n, Grid g_, Consts k_
         nx += gsc * dx;
         ny += gsc * dy;
         nz += gsc * dz;
-        const float r = r2 * __builtin_amdgcn_rsqf(r2);
-        lap += (h - r) * inv_rho;
+        // h - r with one rounding, as the fused XSPH pass computes it; a
+        // distinct particle at the same position (r2 = +0) gives h, as the
+        // reference's w_visc_lap(sqrt(r2)) does (r2 * rsq(r2) would be NaN)
+        const float hr = r2 > 0.0f ? __builtin_fmaf(-r2, __builtin_amdgcn_rsqf(r2), h) : h;
+        lap = __builtin_fmaf(hr, inv_rho, lap);
       }
       return false;
     });

@@ -256,6 +256,43 @@ def test_normal(fx, oracle, scene):
     assert e_n <= TOL and e_k <= 1e-4  # curvature is a ratio of two sums
 
 
+def test_normal_with_coincident_particles(fx, oracle, scene):
+    """Distinct particles at exactly the same position - as two particles
+    clamped into the same corner of the domain end up.  The reference's
+    curvature term for such a pair is w_visc_lap(sqrt(0)) = coef * h
+    (shaders/sph_normal.glsl:73, shaders/sph_common.glsl:121-127): finite.
+    The normal pass matches the oracle on a state with coincident pairs, and
+    cohesion substeps from one (identical positions and velocities, so the
+    pair is still coincident when the XSPH pass computes the next normals)
+    stay finite."""
+    p, st, n = scene["p"], scene["st"], scene["n"]
+    pairs = [(10, 11), (500, 3000), (7000, 7001), (n - 2, n - 1)]
+    pred = st.pred.copy()
+    for a, b in pairs:
+        pred[b] = pred[a]
+    keys, idx, start, end = _stable_grid(oracle, p, pred, n)
+    ref = oracle.normal(p, pred, st.lam, idx, start, end)
+    assert np.isfinite(ref).all()
+    g = GpuState(fx, p, pred=pred, lam=st.lam, sorted_key=keys, sorted_idx=idx)
+    g.run("normal")
+    out = g.get("normal")
+    assert np.isfinite(out).all()
+    assert rel_err(out[:, :3], ref[:, :3], rms(ref[:, :3])) <= TOL
+    assert rel_err(out[:, 3], ref[:, 3], rms(ref[:, 3])) <= 1e-4
+
+    q = scene["cfg"].params(scene["dt"])
+    q.surface_tension = 0.001
+    pos, vel = st.pos.copy(), st.vel.copy()
+    for a, b in pairs:
+        pos[b], vel[b] = pos[a], vel[a]
+    c = GpuState(fx, q, collider=scene["col"], pos=pos, vel=vel)
+    c.run("load_state")
+    for _ in range(3):
+        c.run("substep")
+    for name in ("pos", "vel", "normal"):
+        assert np.isfinite(c.get(name)).all(), name
+
+
 def test_substep_matches_oracle(fx, oracle, scene):
     """One fused substep (fx_substep: normal, predict, grid, 4 x lambda /
     delta+apply, velocity, XSPH, finalize) against the oracle's pass-by-pass
