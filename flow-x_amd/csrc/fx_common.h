@@ -334,7 +334,8 @@ struct Tuning {
   float skin;
   int block;
   int persistent;  // batched passes: workgroups per CU of a persistent launch, 0 = flat
-  int walk;        // large-scene neighbour passes: 0 default walk, 1 row by row, 2 the other stream walk
+  int walk;        // large-scene neighbour passes: 0 default walk, 1 row by row, 2 the other stream walk,
+                   // 3 lambda and delta through LDS tiles of the workgroup's stencil rows
 };
 Tuning tuning();
 

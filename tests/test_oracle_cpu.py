@@ -399,6 +399,21 @@ def test_normal_two_particle_closed_form(fx, oracle):
     p.n = 1
     idx, start, end = _pair_grid(oracle, p, x[:1], 1)
     assert np.all(oracle.normal(p, x[:1], lam[:1], idx, start, end, precision="f64") == 0)
+    # a distinct particle at the same position adds no gradient and the full
+    # h to the Laplacian (w_visc_lap(sqrt(0)) = coef * h), finite in both
+    # precisions
+    p.n = 3
+    x3 = np.concatenate([x, x[:1]])
+    lam3 = np.concatenate([lam, [[870.0, 0, 0, 0]]])
+    idx, start, end = _pair_grid(oracle, p, x3, 3)
+    d = x3[0, :3] - x3[1, :3]
+    r2 = d @ d
+    n = (p.mass / lam3[1, 0]) * (-6.0 * POLY6(h) * (h * h - r2) ** 2) * d
+    lap = (p.mass / lam3[1, 0]) * VLAP(h) * (h - np.sqrt(r2)) + (p.mass / lam3[2, 0]) * VLAP(h) * h
+    out = oracle.normal(p, x3, lam3, idx, start, end, precision="f64")
+    assert np.allclose(out[0, :3], n, rtol=1e-12, atol=0)
+    assert np.isclose(out[0, 3], -lap / np.linalg.norm(n), rtol=1e-12)
+    assert np.isfinite(oracle.normal(p, x3, lam3, idx, start, end)).all()
 
 
 def _cloud(n, seed, lo=-0.3, hi=0.3):
