@@ -517,6 +517,53 @@ def adjoint_summary(fx, substeps=24, reps=3):
             "grad_v0_norm": norm, "gradient_finite": True}
 
 
+def reference_scene_summary(fx, frames=24):
+    """The reference's own scene against its own published figures: the
+    smoke test's default domain at the origin (scripts/smoke_test.py; 2 m
+    cube, default settings, the reference's caps: 15,360 particles,
+    h = 0.1187 m) with its 0.5 m cube collider, `frames` frames of
+    Solver.step (the reference's substeps per frame) and a surface at the
+    default resolution 48 (splat + marching cubes) every frame.  The
+    reference's source quotes ~35 ms per frame for the solver step at 2 m
+    and 15k particles and ~35 ms of CPU marching cubes + ~2 ms of read-back
+    for a resolution-48 surface (solver/surface.py:14-17; hardware not
+    stated).  Timed under the reference's literal update (its semantics; the
+    scene flies apart in its first frame, see the density check) and under
+    the converging update."""
+    import torch
+
+    from flowx_amd.config import REFERENCE_CAPS, DomainSettings, converging_update, resolve_config
+
+    out = {"published_solver_ms_per_frame": 35.0, "published_surface_ms": 37.0,
+           "published_source": "solver/surface.py:14-17 (15k particles, 2 m domain, surface resolution 48)"}
+    for name, literal in (("literal", True), ("converging", False)):
+        settings = DomainSettings()
+        cfg = resolve_config(settings, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), REFERENCE_CAPS)
+        cfg.extra.update(colliders=[("box", (-0.25, -0.25, -0.25), (0.25, 0.25, 0.25))],
+                         collider_resolution=settings.resolution)
+        if not literal:
+            cfg = converging_update(cfg)
+        solver = fx.Solver(cfg, keep_scratch=False)
+        solver.reset()
+        surf = fx.Surface(solver, settings.surface_resolution, settings.surface_iso)
+        frame_dt = 1.0 / 24.0
+        solver.step(frame_dt)  # warm-up frame
+        surf.update()
+        _, t_step = _sync_time(lambda: [solver.step(frame_dt) for _ in range(frames)])
+        _, t_surf = _sync_time(lambda: [surf.update() for _ in range(frames)])
+        steps, _ = cfg.substep_dt(frame_dt)
+        out[name] = {"particles": cfg.particle_count, "substeps_per_frame": steps,
+                     "solver_ms_per_frame": round(t_step * 1e3 / frames, 3),
+                     "surface_ms_per_frame": round(t_surf * 1e3 / frames, 3),
+                     "surface_dims": list(surf.config.dims)}
+        del surf, solver
+        torch.cuda.empty_cache()
+    conv = out["converging"]
+    out["solver_speedup_vs_published"] = round(35.0 / conv["solver_ms_per_frame"], 1)
+    out["surface_speedup_vs_published"] = round(37.0 / conv["surface_ms_per_frame"], 1)
+    return out
+
+
 def scorr_delta(fx, kernels, steps=8):
     """The delta pass with s_corr on, at a strength that still converges on
     the scale scene (k = 1e-5; k >= 1e-4 makes it blow up), timed on its own
@@ -573,8 +620,9 @@ def main():
                     help="the reference's literal position correction (diverges at these masses)")
     ap.add_argument("--no-surface", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="leave out the s_corr, batched and adjoint objects of the default line")
-    ap.add_argument("--workload", default="scale", choices=["scale", "batch", "adjoint", "dambreak", "animated"])
+                    help="leave out the reference-scene, s_corr, batched and adjoint objects of the default line")
+    ap.add_argument("--workload", default="scale",
+                    choices=["scale", "batch", "adjoint", "dambreak", "animated", "reference"])
     ap.add_argument("--frames", type=int, default=None, help="dambreak / animated workloads: frames to run")
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the neighbour-pass tuning on the 1.64M scene, write flow-x_amd/tuning.json, exit")
@@ -617,6 +665,9 @@ def main():
         return
     if args.workload == "dambreak":
         print(json.dumps(dambreak_workload(fx, args.frames or 48, cpu=not args.no_cpu_baseline)))
+        return
+    if args.workload == "reference":
+        print(json.dumps(reference_scene_summary(fx, args.frames or 24)))
         return
     if args.workload == "animated":
         print(json.dumps(animated_workload(fx, args.frames or 24)))
@@ -706,8 +757,10 @@ def main():
 
     del solver
     torch.cuda.empty_cache()
-    scorr = batched = adjoint = None
+    scorr = batched = adjoint = reference = None
     if not args.no_extras:
+        print("[bench] the reference's own scene against its published figures", file=sys.stderr, flush=True)
+        reference = reference_scene_summary(fx)
         print("[bench] s_corr delta pass", file=sys.stderr, flush=True)
         scorr = scorr_delta(fx, kernels)
         print("[bench] 64 batched scenes vs sequential plain runs", file=sys.stderr, flush=True)
@@ -750,6 +803,7 @@ def main():
         "scorr_delta": scorr,
         "batched": batched,
         "adjoint": adjoint,
+        "reference_scene": reference,
     }
     print(json.dumps(out))
 
