@@ -66,6 +66,58 @@ def test_cell_beyond_max_cell_scan(fx, oracle):
     assert e_gpu64 <= max(2 * e_orc64, TOL) and e <= 1e-4
 
 
+def test_cell_beyond_max_cell_scan_large_scene(fx, oracle):
+    """The same 700-particle cell inside a 204,800-particle lattice, where
+    the neighbour passes take the large-scene walks: a lane whose stencil row
+    holds the dense cell leaves the stream / row-batch / LDS-tile walk for the
+    per-cell path that stops each cell's scan after 512 slots.  The lambda
+    pass against the oracle (density bit-exact, lambda within the bar), and
+    two substeps bit-identical across the four walks (FxTuning.walk 0-3)."""
+    import torch
+
+    from flowx_amd import _lib
+    from flowx_amd.config import converging_update
+
+    rng = np.random.default_rng(12)
+    cfg = converging_update(fx.lattice_config((-1.0, -1.0, 0.0), (1.0, 1.0, 2.0), 0.025, (80, 80, 32),
+                                              iterations=4, viscosity=0.01))
+    lattice = fx.seed_positions(cfg)
+    clump = np.concatenate([rng.uniform([0.0026, 0.0026, 0.1026], [0.0224, 0.0224, 0.1224], (700, 3)),
+                            np.ones((700, 1))], 1).astype(np.float32)
+    pos = np.concatenate([lattice, clump])
+    cfg.particle_count = len(pos)
+    _, dt = cfg.substep_dt(1.0 / 24.0)
+    p = cfg.params(dt)
+    n = len(pos)
+    keys, idx, start, end = oracle.build_grid(p, pos, "stable")
+    counts = (end - start)[start >= 0]
+    assert counts.max() > 512 and n > 1 << 17
+    ref = oracle.lambda_(p, pos, idx[:n], start, end)
+    g = GpuState(fx, p, pred=pos, sorted_key=keys[:n], sorted_idx=idx[:n])
+    g.run("lambda")
+    out = g.get("lam")
+    assert np.array_equal(out[:, 0], ref[:, 0])
+    scale = float(np.sqrt(np.mean(ref[:, 1].astype(np.float64) ** 2)))
+    assert rel_err(out[:, 1], ref[:, 1], scale) <= TOL
+    saved = _lib.get_tuning()
+    res = []
+    try:
+        for walk in (0, 1, 2, 3):
+            _lib.set_tuning(walk=walk)
+            s = fx.Solver(cfg, keep_scratch=True)
+            s.reset(pos)
+            s.substeps_n(dt, 2)
+            torch.cuda.synchronize()
+            res.append((s.pos.cpu().numpy(), s.vel.cpu().numpy(), s.lam.cpu().numpy()))
+            del s
+    finally:
+        _lib.set_tuning(**saved)
+    assert np.isfinite(res[0][0]).all()
+    for w, r in enumerate(res[1:], 1):
+        for name, a, b in zip(("pos", "vel", "lam"), res[0], r):
+            assert np.array_equal(a, b), (w, name)
+
+
 def test_splat_dense_cells(fx, oracle):
     """The splat over a lattice plus a 3000-particle clump spread over 2x2x2
     cells (the tiles around it stage more particles than their LDS holds and
