@@ -771,9 +771,9 @@ constexpr int kQuadBelowXsph = 36000;
 // batches within rows and stays ahead (127k: 58 us streamed, 64 by row).
 constexpr int kRowWalk = 64;
 // Kernel template bit: the large-scene passes' other stream walk (the
-// autotuner's walk knob, FxTuning.walk = 2): lambda streams without row
-// batches, delta streams in row batches, XSPH streams with unconditional
-// loads.  Every walk visits the same candidates in the same order.
+// autotuner's walk knob, FxTuning.walk = 2): lambda and delta stream
+// without row batches, XSPH streams with unconditional loads.  Every walk
+// visits the same candidates in the same order.
 constexpr int kWalkAlt = 512;
 constexpr int kRowBelow = 1 << 17;
 
@@ -806,11 +806,14 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
     //   VISIT_MASKED_LOADS: empty slots load nothing (XSPH, whose second
     //     gather already sits behind the range test: 525 vs 575 us).
     //   VISIT_ROW_BATCH: a batch stays inside one row, the row's last batch
-    //     masked, one row check per batch instead of one per slot (lambda).
+    //     masked, one row check per batch instead of one per slot (lambda
+    //     and delta).
     // Per pass, substeps 8-248 of the 1.64M scene (us; row by row / stream /
     // row batches): lambda 351 / 384 / 350, delta 354 / 347 / 360, XSPH
-    // 602 / 525 (masked) / 571.  A lane with a row past max_scan takes the
-    // per-cell path below.
+    // 602 / 525 (masked) / 571.  Once the row batches load from one address
+    // with immediate offsets and take each batch's first slot unconditionally
+    // (below), delta's row batches overtake its stream walk: 306 -> 299 us.
+    // A lane with a row past max_scan takes the per-cell path below.
     // loads in flight per lane (substep at 1.64M particles, batch 3 / 4 / 5 /
     // 6 / 7 / 8: lambda 356 / 346 / 337 / 340 / 338 / 343 us, delta 352 /
     // 344 / 341 / 340 / 342 / 349, XSPH 548 / 530 / 543 / 533 / 575 / 563)
@@ -1618,7 +1621,7 @@ __device__ __forceinline__ void delta_body(int gt_in, int n, Grid g_, Consts k_,
         if constexpr ((V & kLdsWalk) != 0)
           visit_tile<6>(g, off, k.max_scan, bx, by, bz, *tile, ld, body);
         else
-          visit_auto<MODE | ((V & kRowWalk) ? 0 : (V & kWalkAlt) ? VISIT_STREAM | VISIT_ROW_BATCH : VISIT_STREAM)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
+          visit_auto<MODE | ((V & kRowWalk) ? 0 : (V & kWalkAlt) ? VISIT_STREAM : VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0))>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
       }
     }
     const float ox = sum.ox, oy = sum.oy, oz = sum.oz;
