@@ -736,11 +736,16 @@ def main():
         ctr = read_counters(dominant)
         if ctr:
             # the roofline is HBM's (no MFMA work); what the counters show
-            # limits the kernel is the vector-memory pipeline
+            # limits the kernel is each wave's instruction stream and its
+            # dependent loads at the occupancy ceiling, with the
+            # vector-memory pipeline near saturation
             roofline["measured_limiter"] = (
-                f"vector-memory pipeline: TA {ctr.get('ta_busy', 0) * 100:.0f}% / TD {ctr.get('td_busy', 0) * 100:.0f}% "
-                f"busy, VALU lanes {ctr.get('valu_lane_utilisation', 0) * 100:.0f}% active, HBM "
-                f"{roofline['frac'] * 100:.1f}% of peak (committed rocprofv3 counters)")
+                f"per-wave instruction issue and load latency at 8 waves per SIMD: VALU "
+                f"{ctr.get('valu_insts_per_cu_cycle', 0):.2f} instructions per CU per cycle, "
+                f"{ctr.get('wait_inst_share', 0) * 100:.0f}% of wave cycles waiting on a dependency, TA "
+                f"{ctr.get('ta_busy', 0) * 100:.0f}% / TD {ctr.get('td_busy', 0) * 100:.0f}% busy, VALU lanes "
+                f"{ctr.get('valu_lane_utilisation', 0) * 100:.0f}% active, HBM {roofline['frac'] * 100:.1f}% of "
+                f"peak (committed rocprofv3 counters)")
 
     surface = None
     if not args.no_surface:
