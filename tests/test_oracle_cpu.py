@@ -563,7 +563,7 @@ def test_splat_closed_form_brute_force_and_carve_out(fx, oracle):
 
 # ---- configuration and substep sizing (solver/sph.py:228-313) ----------------
 
-def test_substep_sizing_and_capped_grids_match_the_survey_numbers(fx):
+def test_substep_sizing_capped_grids_and_coarsening_match_hand_derived_numbers(fx):
     """Substeps per frame and dt at 24 fps for the benchmark configurations,
     the cell grids the reference's caps (MAX_CELLS 262144, 128 per axis)
     give at h = 0.1 / 0.025 / 0.018 (cells grown 1.5x until they fit: 20^3,
