@@ -4,11 +4,15 @@ No reference counterpart.  Sweeps the tuning knobs of include/flowx.h
 (FxTuning): the block size of the neighbour passes, the neighbour-list
 skin, the stencil walk of a large scene's neighbour passes (how the
 stencil's cell rows are staged into batches of loads in flight, or into LDS
-tiles per workgroup) on a plain scene, and the workgroups per CU of the batched passes' persistent launch
-(0 = flat launch) on a batch of scenes.  Each setting is timed with HIP
-events over a few substeps after a warm-up, and every setting must give the
-same positions as the first (the knobs change launch shape and candidate
-iteration, never results).  The fastest setting is written to a JSON file
+tiles per workgroup) on a plain scene, and the workgroups per CU of the
+batched passes' persistent launch (0 = flat launch) on a batch of scenes.
+Each setting is timed with HIP events after a warm-up - the plain scene
+over a few frames, each one call of the frame's substeps (the first enters
+from the canonical state, the rest in sorted order, as playback runs them;
+single-substep calls favour other block sizes), the batch over a few
+batched substeps - and every setting must give the same positions as the
+first (the knobs change launch shape and candidate iteration, never
+results).  The fastest setting is written to a JSON file
 that ``apply()`` loads - by default ``tuning.json`` next to the library,
 read by ``Solver`` and ``BatchSolver`` at construction when present.
 ``python bench.py --autotune`` tunes on the 1.64M-particle scene and the
@@ -48,7 +52,7 @@ def _timed(run, setting, warmup, steps):
     import torch
 
     _lib.set_tuning(**setting)
-    state, step = run()
+    state, step, per_step = run()
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
@@ -58,7 +62,7 @@ def _timed(run, setting, warmup, steps):
         step()
     e1.record()
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
+    ms = e0.elapsed_time(e1) / (steps * per_step)
     pos = state().clone()
     torch.cuda.empty_cache()
     return ms, pos
@@ -87,12 +91,14 @@ def tune(fx, cfg=None, blocks=BLOCKS, skins=SKINS, walks=WALKS, batch_cfgs=None,
     write the results and the fastest settings to `path`."""
     cfg = cfg or fx.scale_config()
     saved = _lib.get_tuning()
-    _, dt = cfg.substep_dt(1.0 / cfg.extra.get("fps", 24.0))
+    frame, dt = cfg.substep_dt(1.0 / cfg.extra.get("fps", 24.0))
 
     def plain():
+        # a frame's substeps per call, as a playback frame runs them: the
+        # first enters from the canonical state, the rest in sorted order
         solver = fx.Solver(cfg, keep_scratch=False)
         solver.reset()
-        return (lambda: solver.pos), (lambda: solver.substeps_n(dt, 1))
+        return (lambda: solver.pos), (lambda: solver.substeps_n(dt, frame)), frame
 
     try:
         base = {"persistent": 0}
@@ -106,7 +112,7 @@ def tune(fx, cfg=None, blocks=BLOCKS, skins=SKINS, walks=WALKS, batch_cfgs=None,
 
             def batched():
                 b = fx.BatchSolver(cfgs)
-                return (lambda: b.pos), b.substep
+                return (lambda: b.pos), b.substep, 1
 
             batch_settings = [{"block": best["block"], "list_skin": best["list_skin"], "walk": best["walk"],
                                "persistent": k} for k in persistents]
