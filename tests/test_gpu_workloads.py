@@ -5,8 +5,9 @@
   cell ranges bit-exact; predict and finalize bit-exact; lambda, delta,
   XSPH and normal within 1e-5; one fused substep within 1e-5 h or, where
   the oracle's own float32 result moves further when only the order of its
-  sums or its precision changes, within that distance; the GPU cylinder +
-  box voxelizer bit-equal to the voxel-centre test.
+  sums or its precision changes, within that distance; three frames of it
+  against the oracle by density-error, height and speed statistics; the GPU
+  cylinder + box voxelizer bit-equal to the voxel-centre test.
 * The 4.1M-particle animated scene: the sphere voxelizer at 111^3 over the
   drop frames, the grid build and a lambda pass against the oracle, one
   fused substep within 1e-5 h.
@@ -221,6 +222,40 @@ def test_scale_fused_substep(fx, oracle, scale, sigma):
         assert e_vel <= max(10 * TOL, e_vel_order)
     else:
         assert e_pos <= TOL or e64_gpu <= 2 * e64_orc
+
+
+def test_scale_run_statistics(fx, oracle, scale):
+    """Three frames (12 substeps, cohesion on, one fx_step call: the resident
+    hand-off between substeps) of the 1.64M scene from the developed state,
+    against the float32 oracle in the reference's bitonic order: the run is
+    chaotic, so it is compared as long runs are, by the density-constraint
+    error and fluid-height statistics - mean density and
+    mean |rho / rho0 - 1| within 5e-3, mean and 90th-percentile height
+    within 0.02 / 0.05 h, rms speed within 1%."""
+    import torch
+
+    cfg, p, st, col = scale["cfg"], scale["p"], scale["st"], scale["col"]
+    steps = 12
+    ref = oracle.State(p, st.pos, st.vel, precision="f32", sort_mode="bitonic")
+    for _ in range(steps):
+        ref.substep(col)
+    solver = fx.Solver(cfg, keep_scratch=True)
+    solver.reset(st.pos)
+    solver.load_state(st.pos, st.vel)
+    solver.substeps_n(scale["dt"], steps)
+    torch.cuda.synchronize()
+    gp, gv, gl = solver.read_positions(), solver.read_velocities(), solver.lam.cpu().numpy()
+    h, rho0 = cfg.smoothing_radius, p.rest_density
+    d_rho = abs(gl[:, 0].mean() - ref.lam[:, 0].mean()) / rho0
+    d_err = abs(np.abs(gl[:, 0] / rho0 - 1).mean() - np.abs(ref.lam[:, 0] / rho0 - 1).mean())
+    d_zmean = abs(gp[:, 2].mean() - ref.pos[:, 2].mean()) / h
+    d_z90 = abs(np.percentile(gp[:, 2], 90) - np.percentile(ref.pos[:, 2], 90)) / h
+    v_gpu, v_ref = rms(gv[:, :3]), rms(ref.vel[:, :3])
+    print(f"scale, {steps} substeps: mean rho {gl[:, 0].mean():.2f} / {ref.lam[:, 0].mean():.2f}, |C| diff "
+          f"{d_err:.2e}, mean height diff {d_zmean:.2e} h, p90 height diff {d_z90:.2e} h, rms speed "
+          f"{v_gpu:.4f} / {v_ref:.4f}")
+    assert d_rho < 5e-3 and d_err < 5e-3 and d_zmean < 2e-2 and d_z90 < 5e-2
+    assert abs(v_gpu - v_ref) <= 0.01 * v_ref
 
 
 # ---- the animated scene ------------------------------------------------------------
