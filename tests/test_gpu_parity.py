@@ -252,8 +252,15 @@ def test_normal(fx, oracle, scene):
     e_n = rel_err(out[:, :3], ref[:, :3], ns)
     ks = rms(ref64[:, 3])
     e_k = rel_err(out[:, 3], ref[:, 3], ks)
-    print(f"normal: err {e_n:.2e}, curvature err {e_k:.2e}")
-    assert e_n <= TOL and e_k <= 1e-4  # curvature is a ratio of two sums
+    # curvature is the ratio -lap / |n|, and |n| is a sum that cancels inside
+    # the fluid: its float32 value carries the cancellation's rounding, so it
+    # is held to the float64 oracle, no further than twice the float32
+    # oracle's own error (or 1e-5)
+    k_gpu = rel_err(out[:, 3], ref64[:, 3], ks)
+    k_orc = rel_err(ref[:, 3], ref64[:, 3], ks)
+    print(f"normal: err {e_n:.2e}, curvature err {e_k:.2e} (vs f64: gpu {k_gpu:.2e}, oracle f32 {k_orc:.2e})")
+    assert e_n <= TOL
+    assert e_k <= TOL or k_gpu <= 2 * k_orc
 
 
 def test_normal_with_coincident_particles(fx, oracle, scene):
@@ -278,7 +285,10 @@ def test_normal_with_coincident_particles(fx, oracle, scene):
     out = g.get("normal")
     assert np.isfinite(out).all()
     assert rel_err(out[:, :3], ref[:, :3], rms(ref[:, :3])) <= TOL
-    assert rel_err(out[:, 3], ref[:, 3], rms(ref[:, 3])) <= 1e-4
+    ref64 = oracle.normal(p, pred, st.lam, idx, start, end, precision="f64")
+    ks = rms(ref64[:, 3])
+    assert rel_err(out[:, 3], ref[:, 3], ks) <= TOL or \
+        rel_err(out[:, 3], ref64[:, 3], ks) <= 2 * rel_err(ref[:, 3], ref64[:, 3], ks)
 
     q = scene["cfg"].params(scene["dt"])
     q.surface_tension = 0.001
@@ -344,11 +354,19 @@ def test_literal_update_first_substep(fx, oracle, scene):
     g = GpuState(fx, p, collider=scene["col"], pos=pos0)
     g.run("load_state")
     g.run("substep")
-    e_pos = rel_err(g.get("pos")[:, :3], ref.pos[:, :3], cfg.smoothing_radius)
+    ref64 = oracle.State(p, pos0, precision="f64", sort_mode="stable")
+    ref64.substep(scene["col"])
+    h = cfg.smoothing_radius
+    e_pos = rel_err(g.get("pos")[:, :3], ref.pos[:, :3], h)
     e_rho = rel_err(g.get("lam")[:, 0], ref.lam[:, 0], p.rest_density)
-    print(f"literal update, 1 substep: pos err {e_pos:.2e}, rho err {e_rho:.2e}; "
-          f"oracle mean rho {ref.lam[:, 0].mean():.1f}")
-    assert e_pos <= 1e-4 and e_rho <= 1e-4
+    e64_gpu = rel_err(g.get("pos")[:, :3], ref64.pos[:, :3], h)
+    e64_orc = rel_err(ref.pos[:, :3], ref64.pos[:, :3], h)
+    print(f"literal update, 1 substep: pos err {e_pos:.2e} (vs f64: gpu {e64_gpu:.2e}, oracle f32 {e64_orc:.2e}), "
+          f"rho err {e_rho:.2e}; oracle mean rho {ref.lam[:, 0].mean():.1f}")
+    # the density of the substep's lambda pass is bit-exact; positions within
+    # 1e-5 h, or no further from float64 than twice the float32 oracle
+    assert e_rho <= TOL
+    assert e_pos <= TOL or e64_gpu <= 2 * e64_orc
 
 
 def test_substep_deterministic(fx, scene):
