@@ -629,6 +629,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run three plain device-to-device copies of MIB MiB (the reference "
                          "dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--walk", type=int, default=None, choices=[0, 1, 2, 3],
+                    help="scale scene: run with this large-scene walk (FxTuning.walk; 3 = LDS tiles)")
+    ap.add_argument("--block", type=int, default=None, choices=[64, 128, 256],
+                    help="scale scene: run with this neighbour-pass block size (FxTuning.block)")
     ap.add_argument("--fd-directions", type=int, default=32)
     ap.add_argument("--fd-checks", default="4:1e-4:1e-8,8:1e-6:1e-9",
                     help="adjoint workload: comma-separated substeps:eps:eps_translation finite-difference checks")
@@ -679,6 +683,8 @@ def main():
 
     cfg = fx.scale_config(literal_update=args.literal_update)
     solver = fx.Solver(cfg, keep_scratch=False)
+    if args.walk is not None or args.block is not None:  # after the Solver applied any tuning file
+        _lib.set_tuning(walk=args.walk, block=args.block)
     solver.reset()
     _, dt = cfg.substep_dt(1.0 / cfg.extra["fps"])
     n, c = cfg.particle_count, cfg.cell_count
@@ -799,6 +805,7 @@ def main():
             "scorr_k": cfg.scorr_strength,
             "particles": n, "cells": c, "dt": dt,
         },
+        "tuning": _lib.get_tuning(),
         "roofline": roofline,
         "neighbour_passes": neighbour_passes,
         "streaming": streaming_rooflines(kernels, n, c),
