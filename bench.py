@@ -629,8 +629,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run three plain device-to-device copies of MIB MiB (the reference "
                          "dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
-    ap.add_argument("--walk", type=int, default=None, choices=[0, 1, 2, 3],
-                    help="scale scene: run with this large-scene walk (FxTuning.walk; 3 = LDS tiles)")
+    ap.add_argument("--walk", type=int, default=None, choices=[0, 1, 2, 3, 4],
+                    help="scale scene: run with this large-scene walk (FxTuning.walk; 3 = LDS tiles, "
+                         "4 = the default walk without lambda's pair masks for delta)")
     ap.add_argument("--block", type=int, default=None, choices=[64, 128, 256],
                     help="scale scene: run with this neighbour-pass block size (FxTuning.block)")
     ap.add_argument("--fd-directions", type=int, default=32)

@@ -4,8 +4,9 @@ No reference counterpart.  Sweeps the tuning knobs of include/flowx.h
 (FxTuning): the block size of the neighbour passes, the neighbour-list
 skin, the stencil walk of a large scene's neighbour passes (how the
 stencil's cell rows are staged into batches of loads in flight, or into LDS
-tiles per workgroup) on a plain scene, and the workgroups per CU of the
-batched passes' persistent launch (0 = flat launch) on a batch of scenes.
+tiles per workgroup, and whether delta walks the pair masks lambda
+records) on a plain scene, and the workgroups per CU of the batched
+passes' persistent launch (0 = flat launch) on a batch of scenes.
 Each setting is timed with HIP events after a warm-up - the plain scene
 over a few frames, each one call of the frame's substeps (the first enters
 from the canonical state, the rest in sorted order, as playback runs them;
@@ -28,7 +29,7 @@ from . import _lib
 DEFAULT_PATH = Path(__file__).resolve().parent / "tuning.json"
 BLOCKS = (64, 128, 256)
 SKINS = (0.0, 0.1, 0.2, 0.3)
-WALKS = (0, 1, 2, 3)
+WALKS = (0, 1, 2, 3, 4)
 PERSISTENT = (0, 8, 16, 24, 32)
 KNOBS = ("block", "list_skin", "walk", "persistent")
 

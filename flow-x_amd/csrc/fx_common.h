@@ -296,6 +296,12 @@ struct ProfScope {
     ::fx::ProfScope prof_(#kernel, st);                         \
     hipLaunchKernelGGL(kernel, grid, block, 0, st, __VA_ARGS__); \
   } while (0)
+// the same with `shm` bytes of dynamic LDS per workgroup
+#define FX_LAUNCH_SHM(kernel, grid, block, shm, st, ...)           \
+  do {                                                            \
+    ::fx::ProfScope prof_(#kernel, st);                           \
+    hipLaunchKernelGGL(kernel, grid, block, shm, st, __VA_ARGS__); \
+  } while (0)
 
 // Workgroups are dispatched round-robin over the chip's 8 XCDs, each with
 // its own L2.  xcd_block() renumbers the blocks of a 1-D launch so that each
@@ -335,7 +341,8 @@ struct Tuning {
   int block;
   int persistent;  // batched passes: workgroups per CU of a persistent launch, 0 = flat
   int walk;        // large-scene neighbour passes: 0 default walk, 1 row by row, 2 the other stream walk,
-                   // 3 lambda and delta through LDS tiles of the workgroup's stencil rows
+                   // 3 lambda and delta through LDS tiles of the workgroup's stencil rows, 4 the default
+                   // walk without the pair masks lambda records for delta
 };
 Tuning tuning();
 

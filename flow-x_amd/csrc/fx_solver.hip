@@ -1035,6 +1035,132 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
   }
 }
 
+// ---- pair masks: lambda records, delta walks --------------------------------
+// The delta pass of a Jacobi iteration visits the candidates lambda visited,
+// at the same positions, and keeps the same ones (r^2 < h^2; delta's r^2 >
+// 1e-12 test is made again on the candidates themselves).  Lambda therefore
+// records, per stencil row r, a 64-bit word whose bit j says that slot
+// S[r] + j (S[r]: the row's first slot) lay within h; delta then loads only
+// those ~34 of the ~216 candidates, in slot order, so its sums see the same
+// terms in the same order and are bit-identical to a full walk.  A particle
+// with a row of more than 64 slots (or past max_scan) is not recorded
+// (whole[t] = 1) and delta walks its full stencil.  The words live in the
+// neighbour-list mask area of the workspace ([9][n] words), which is unused
+// while the lists are off; the resident hand-off state that shares it is dead
+// between the gather and the substep's final finalize.  Lambda keeps a row's
+// word in registers while it walks the row and parks it in LDS at the row's
+// end (LDS stores are not counted with the loads in flight), then writes the
+// nine words out once.  On the 1.64M scene: delta 302 -> 151 us, lambda 314
+// -> 330 us (the recording's registers), substep 3.09 -> 2.56 ms.
+struct PairMasks {
+  uint64_t* rows;  // [9][n]
+  int32_t* whole;  // [n]: 1 = not recorded, walk the full stencil
+};
+constexpr int kMaskMaxRow = 64;
+
+// The row-batch walk of visit (VISIT_ROW_BATCH, row lengths packed two to a
+// register), recording each row's in-range bits: f returns whether the
+// candidate lies within h.  rm: this lane's first row word in LDS, `stride`
+// words between rows.  Returns false, having visited nothing, when a row is
+// longer than kMaskMaxRow or max_scan.
+template <int B, class LD, class F>
+__device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __restrict__ off, int max_scan,
+                                             int bx, int by, int bz, LD&& ld, F&& f, uint64_t* rm, int stride) {
+  static_assert(B <= kCandPad, "row batches read up to B - 1 slots past a run");
+  const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
+  int S[9];
+  uint32_t Lp[5];
+  bool big = false;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
+    const bool in = z >= 0 && z < g.cz && y >= 0 && y < g.cy;
+    const int c0 = in ? (z * g.cy + y) * g.cx + x0 : 0;
+    S[r] = in ? off[c0] : 0;
+    const int len = in ? off[c0 + (x1 - x0) + 1] - S[r] : 0;
+    big |= len > max_scan || len > kMaskMaxRow;
+    if (r % 2 == 0)
+      Lp[r / 2] = (uint32_t)len;
+    else
+      Lp[r / 2] |= (uint32_t)len << 16;
+  }
+  if (big) return false;
+  int k = S[0], e = S[0] + (int)(Lp[0] & 0xFFFFu), left = 8;
+  uint64_t acc = 0;
+  auto next_row = [&]() {
+    while (k >= e && left > 0) {
+      rm[(8 - left) * stride] = acc;
+      acc = 0;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) S[r] = S[r + 1];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Lp[i] = (Lp[i] >> 16) | (Lp[i + 1] << 16);
+      Lp[4] >>= 16;
+      k = S[0];
+      e = S[0] + (int)(Lp[0] & 0xFFFFu);
+      --left;
+    }
+  };
+  next_row();
+#pragma unroll 1
+  while (k < e) {
+    decltype(ld(0)) d[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) d[u] = ld(k + u);
+    const int rem = e - k;
+    uint32_t bits = f(k, d[0]) ? 1u : 0u;
+#pragma unroll
+    for (int u = 1; u < B; ++u)
+      if (u < rem && f(k + u, d[u])) bits |= 1u << u;
+    acc |= (uint64_t)bits << (k - S[0]);
+    k += B;
+    next_row();
+  }
+  rm[(8 - left) * stride] = acc;
+  return true;
+}
+
+// Delta's walk over the recorded words: per row, B set bits at a time (the
+// batch's loads issued together, empty slots re-loading its first), the
+// candidates reaching f in slot order.
+template <int B, class LD, class F>
+__device__ __forceinline__ void visit_masked(const Grid& g, const int32_t* __restrict__ off, int bx, int by,
+                                             int bz, int t, int n, const uint64_t* __restrict__ rows, LD&& ld,
+                                             F&& f) {
+  const int x0 = max(bx - 1, 0);
+  int S[9];
+  uint64_t M[9];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
+    const bool in = z >= 0 && z < g.cz && y >= 0 && y < g.cy;
+    S[r] = in ? off[(z * g.cy + y) * g.cx + x0] : 0;
+    M[r] = rows[(size_t)r * n + t];
+  }
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    uint64_t m = M[r];
+    const int s = S[r];
+#pragma unroll 1
+    while (m) {
+      int q[B];
+      bool v[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        v[u] = m != 0;
+        q[u] = s + __ffsll((unsigned long long)m) - 1;  // (s - 1 once m is empty: not loaded)
+        m &= m - 1;
+      }
+      decltype(ld(0)) d[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) d[u] = ld(v[u] ? q[u] : q[0]);
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (v[u]) f(q[u], d[u]);
+    }
+  }
+}
+
 // Run `body` over the candidates with the lists when they are valid for this
 // particle, else over the full stencil.
 template <int MODE, class LD, class F, class RF = NoRefine>
@@ -1083,6 +1209,11 @@ __device__ __forceinline__ void visit_auto(const Grid& g, const int32_t* __restr
 // 389 against 69 / 168 / 324.  Kept as a walk the autotuner measures
 // (bit-identical to the others); it has not won on a scene measured.
 constexpr int kLdsWalk = 1024;
+// Kernel template bit (pair masks, see PairMasks): lambda records its
+// in-range candidates per stencil row, and the delta pass of the same
+// iteration walks only those.
+constexpr int kMaskRecord = 2048;
+constexpr int kMaskWalk = 2048;
 constexpr int kTileBlock = 256;
 constexpr int kTileCap = 2496;  // float4 slots: four workgroups share a CU's 160 KB
 constexpr int kTileGlobal = std::numeric_limits<int>::min();
@@ -1458,9 +1589,11 @@ template <int V>
 __device__ __forceinline__ void lambda_body(int gt_in, int n, Grid g_, Consts k_, float4* __restrict__ pred_s,
                                             const int32_t* __restrict__ off_, float2* __restrict__ dl_s,
                                             const int32_t* __restrict__ sidx, float4* __restrict__ lambda4, Lists L,
-                                            Scenes S, int iter, const Tile* tile = nullptr) {
+                                            Scenes S, int iter, const Tile* tile = nullptr,
+                                            const PairMasks& pm = PairMasks{nullptr, nullptr}, uint64_t* rm = nullptr) {
   constexpr int MODE = V & 3;
   constexpr bool QUAD = (V & kQuadWalk) != 0;
+  constexpr bool REC = (V & kMaskRecord) != 0;
   const int gt = gt_in;
   const int t = QUAD ? gt >> 2 : gt;
   if (t >= n) return;
@@ -1470,7 +1603,7 @@ __device__ __forceinline__ void lambda_body(int gt_in, int n, Grid g_, Consts k_
   if (V & 8) {
     const SceneDev& sc = S.s[scene_of(S, t)];
     // batched: a scene past its own iteration count keeps its last lambda
-    // pass (its densities feed XSPH)
+    // pass (its densities feed XSPH; its delta pass reads no words)
     if (iter >= sc.p.iterations) return;
     g = sc.g;
     k = sc.k;
@@ -1505,7 +1638,26 @@ __device__ __forceinline__ void lambda_body(int gt_in, int n, Grid g_, Consts k_
         }
         return r2 < r2l;
       };
-      if constexpr ((V & kLdsWalk) != 0)
+      if constexpr (REC) {
+        // the same body; what it returns is the pair-mask bit (within h)
+        auto body_rec = [&](int q, const float4& pj) -> bool {
+          const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+          const float r2 = dist2_exact(dx, dy, dz);
+          if (r2 < h2) {
+            sum.add_dens(lam_dens(k, r2));
+            if (r2 > 1e-12f) sum.add_grad(lam_grad(0.0f, lam_coef(k, r2), dx, dy, dz));
+          }
+          return r2 < h2;
+        };
+        const bool recorded = visit_record<5>(g, off, k.max_scan, bx, by, bz, ld, body_rec, rm, (int)blockDim.x);
+        if (recorded) {
+#pragma unroll
+          for (int r = 0; r < 9; ++r) pm.rows[(size_t)r * n + t] = rm[r * (int)blockDim.x];
+        } else {
+          visit<VISIT_FULL>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
+        }
+        pm.whole[t] = recorded ? 0 : 1;
+      } else if constexpr ((V & kLdsWalk) != 0)
         visit_tile<6>(g, off, k.max_scan, bx, by, bz, *tile, ld, body);
       else
         visit_auto<MODE | ((V & kRowWalk) ? 0 : (V & kWalkAlt) ? VISIT_STREAM : VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0))>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
@@ -1533,8 +1685,24 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
                                                 float2* __restrict__ dl_s,
                                                 const int32_t* __restrict__ sidx,
                                                 float4* __restrict__ lambda4, Lists L, Scenes S,
-                                                int iter = -1, const int32_t* __restrict__ tkey = nullptr) {
-  if constexpr ((V & kPersistent) != 0) {
+                                                int iter = -1, const int32_t* __restrict__ tkey = nullptr,
+                                                PairMasks pm = PairMasks{nullptr, nullptr}) {
+  if constexpr ((V & kMaskRecord) != 0) {
+    // each lane's nine row words in LDS, row r at rm[r * blockDim.x]
+    // (launched with 9 * 8 * blockDim.x bytes of dynamic LDS)
+    extern __shared__ uint64_t fx_row_words[];
+    uint64_t* rm = fx_row_words + threadIdx.x;
+    if constexpr ((V & kPersistent) != 0) {
+      constexpr bool QUADW = (V & kQuadWalk) != 0;
+      const long long nth = (QUADW ? 4LL * n : (long long)n);
+      for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
+        lambda_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, g_, k_, pred_s, off_, dl_s, sidx, lambda4, L, S,
+                       iter, nullptr, pm, rm);
+    } else {
+      lambda_body<V>(blockIdx.x * blockDim.x + threadIdx.x, n, g_, k_, pred_s, off_, dl_s, sidx, lambda4, L, S, iter,
+                     nullptr, pm, rm);
+    }
+  } else if constexpr ((V & kPersistent) != 0) {
     constexpr bool QUADW = (V & kQuadWalk) != 0;
     const long long nth = (QUADW ? 4LL * n : (long long)n);
     for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
@@ -1561,7 +1729,8 @@ __device__ __forceinline__ void delta_body(int gt_in, int n, Grid g_, Consts k_,
                                            float4* __restrict__ pred_out, const float4* __restrict__ pos_s,
                                            float4* __restrict__ vel_s, const int32_t* __restrict__ sidx,
                                            float4* __restrict__ delta4, Lists L, const int32_t* __restrict__ skey,
-                                           Scenes S, int iter, const Tile* tile = nullptr) {
+                                           Scenes S, int iter, const Tile* tile = nullptr,
+                                           const PairMasks& pm = PairMasks{nullptr, nullptr}) {
   constexpr int MODE = V & 3;
   constexpr bool QUAD = (V & kQuadWalk) != 0;
   const int gt = gt_in;
@@ -1619,7 +1788,12 @@ __device__ __forceinline__ void delta_body(int gt_in, int n, Grid g_, Consts k_,
           if (r2 < h2 && r2 > 1e-12f) sum.add(DelTerm{del_coef(k, li, pj.w, inv_wq, scorr, r2), dx, dy, dz});
           return false;
         };
-        if constexpr ((V & kLdsWalk) != 0)
+        if constexpr ((V & kMaskWalk) != 0) {
+          if (pm.whole[t] == 0)
+            visit_masked<4>(g, off, bx, by, bz, t, n, pm.rows, ld, body);
+          else
+            visit<VISIT_FULL>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
+        } else if constexpr ((V & kLdsWalk) != 0)
           visit_tile<6>(g, off, k.max_scan, bx, by, bz, *tile, ld, body);
         else
           visit_auto<MODE | ((V & kRowWalk) ? 0 : (V & kWalkAlt) ? VISIT_STREAM : VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0))>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
@@ -1657,12 +1831,13 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
                                                const int32_t* __restrict__ sidx,
                                                float4* __restrict__ delta4, Lists L,
                                                const int32_t* __restrict__ skey, Scenes S,
-                                               int iter = -1, const int32_t* __restrict__ tkey = nullptr) {
+                                               int iter = -1, const int32_t* __restrict__ tkey = nullptr,
+                                               PairMasks pm = PairMasks{nullptr, nullptr}) {
   if constexpr ((V & kPersistent) != 0) {
     constexpr bool QUADW = (V & kQuadWalk) != 0;
     const long long nth = (QUADW ? 4LL * n : (long long)n);
     for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
-      delta_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter);
+      delta_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter, nullptr, pm);
   } else if constexpr ((V & kLdsWalk) != 0) {
     // launched with kTileBlock threads; tkey: the sorted build keys
     __shared__ Tile tile;
@@ -1670,7 +1845,7 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
     stage_tile(tile, g_, off_, tkey, pred_in, t0, n);
     delta_body<V>(t0 + (int)threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter, &tile);
   } else {
-    delta_body<V>(blockIdx.x * blockDim.x + threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter);
+    delta_body<V>(blockIdx.x * blockDim.x + threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter, nullptr, pm);
   }
 }
 
@@ -2214,6 +2389,13 @@ static Lists make_lists(const Ws& w, const Consts& k, bool enabled) {
   return L;
 }
 
+// Pair masks (PairMasks) serve the large-scene walks without neighbour lists,
+// unless the tuning asks for the walks without them (walk 4).  Their words
+// take the lists' mask area and whole-stencil flags.
+static bool pair_masks_on(long long n, bool lists) { return !lists && g_tuning.walk == 0 && n >= kRowBelow; }
+static PairMasks pair_masks(const Ws& w) { return PairMasks{w.mask, w.lfull}; }
+static size_t row_words_lds(int block) { return (size_t)9 * sizeof(uint64_t) * (size_t)block; }
+
 static int fill(int32_t* a, long long n, int32_t v, hipStream_t st) {
   if (n <= 0) return FX_OK;
   LAUNCH(k_fill_i32, n, st, a, n, v);
@@ -2368,10 +2550,16 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
     FX_TRY(fill(w.dcell, p.n_cells, 0, st));
     FX_TRY(fill(w.flags + 2, 1, 0, st));
   }
+  const bool masks = pair_masks_on(n, lists);
+  const PairMasks pm = pair_masks(w);
   for (int it = 0; it < iters; ++it) {
     const bool last = it == iters - 1;
     float4* lam_out = (last && last_in_call && s.lambda) ? (float4*)s.lambda : (float4*)nullptr;
-    if (!lists && g_tuning.walk == 3 && n >= kRowBelow)
+    if (masks)
+      FX_LAUNCH_SHM(k_lambda<VISIT_FULL + kMaskRecord>, dim3(grid_for(n, g_tuning.block)), dim3(g_tuning.block),
+                    row_words_lds(g_tuning.block), st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L,
+                    Scenes{nullptr, 0}, -1, (const int32_t*)nullptr, pm);
+    else if (!lists && g_tuning.walk == 3 && n >= kRowBelow)
       FX_LAUNCH(k_lambda<VISIT_FULL + kLdsWalk>, dim3(grid_for(n, kTileBlock)), dim3(kTileBlock), st, n, g, k,
                 w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0}, -1, (const int32_t*)w.skey);
     else if (!lists)
@@ -2380,7 +2568,11 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
       NLAUNCH(k_lambda<VISIT_BUILD>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else
       NLAUNCH(k_lambda<VISIT_LIST>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
-    if (lists)
+    if (masks)
+      NLAUNCH(k_delta<VISIT_FULL + kMaskWalk>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s,
+              w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
+              (float4*)nullptr, L, (const int32_t*)nullptr, Scenes{nullptr, 0}, -1, (const int32_t*)nullptr, pm);
+    else if (lists)
       NLAUNCH(k_delta<VISIT_LIST>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
               w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
               (float4*)nullptr, L, (const int32_t*)w.skey, Scenes{nullptr, 0});
@@ -2748,7 +2940,7 @@ int fx_set_tuning(const FxTuning* t) {
     return fail(FX_EINVAL, "fx_set_tuning: block %d not in {64, 128, 256}", t->block);
   if (t->persistent < 0 || t->persistent > 32)
     return fail(FX_EINVAL, "fx_set_tuning: persistent %d not in [0, 32]", t->persistent);
-  if (t->walk < 0 || t->walk > 3) return fail(FX_EINVAL, "fx_set_tuning: walk %d not in {0, 1, 2, 3}", t->walk);
+  if (t->walk < 0 || t->walk > 4) return fail(FX_EINVAL, "fx_set_tuning: walk %d not in {0, 1, 2, 3, 4}", t->walk);
   g_tuning.skin = t->list_skin;
   g_tuning.block = t->block;
   g_tuning.persistent = t->persistent;

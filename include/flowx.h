@@ -153,9 +153,12 @@ int fx_profile(int enable);
  * the autotuner.  list_skin: neighbour-list margin in units of h (0 = always
  * scan the full 27-cell stencil); block: 64, 128 or 256 threads; walk: how a
  * large scene's neighbour passes stage the stencil's cell rows into batches
- * of loads in flight (0: per pass default, 1: row by row, 2: the other stream
- * walk, 3: lambda and delta read their candidates from LDS tiles of the
- * workgroup's stencil rows, workgroups of 256 particles); persistent: the batched passes' launch, 0 = one workgroup per block
+ * of loads in flight (0: per pass default, with lambda recording which
+ * candidates lie within h and the same iteration's delta visiting only
+ * those, 1: row by row, 2: the other stream walk, 3: lambda and delta read
+ * their candidates from LDS tiles of the workgroup's stencil rows,
+ * workgroups of 256 particles, 4: the per pass default without the recorded
+ * candidates); persistent: the batched passes' launch, 0 = one workgroup per block
  * of particles, k > 0 = k workgroups per CU looping over them (XCD-contiguous
  * ranges).  Results do not depend on any of them. */
 typedef struct FxTuning {

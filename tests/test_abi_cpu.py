@@ -104,7 +104,7 @@ def test_tuning_round_trip_and_rejects_bad_values(fx):
         _lib.set_tuning(persistent=4, walk=3)
         t = _lib.get_tuning()
         assert t["persistent"] == 4 and t["walk"] == 3
-        for bad in ({"block": 96}, {"list_skin": -0.5}, {"list_skin": 2.0}, {"persistent": -1}, {"walk": 4},
+        for bad in ({"block": 96}, {"list_skin": -0.5}, {"list_skin": 2.0}, {"persistent": -1}, {"walk": 5},
                     {"walk": -1}):
             with pytest.raises(RuntimeError):
                 _lib.set_tuning(**bad)

@@ -218,9 +218,9 @@ def test_autotune_sweep_keeps_results_identical(fx, tmp_path):
         cfg = converging_update(fx.lattice_config((-1.0, -1.0, 0.0), (1.0, 1.0, 2.0), 0.025, (80, 80, 32),
                                                   iterations=4, viscosity=0.01, surface_tension=0.001))
         cfg.extra.update(colliders=[("box", (-0.2, -0.2, 0.0), (0.2, 0.2, 0.3))], collider_resolution=40)
-        out = autotune.tune(fx, cfg, blocks=(64, 256), skins=(0.0, 0.2), walks=(0, 1, 2, 3), batch_cfgs=_scenes(fx),
+        out = autotune.tune(fx, cfg, blocks=(64, 256), skins=(0.0, 0.2), walks=(0, 1, 2, 3, 4), batch_cfgs=_scenes(fx),
                             persistents=(0, 2, 8), warmup=1, steps=2, path=tmp_path / "t.json")
-        assert len(out["results"]) == 10 and all(r["same_result"] for r in out["results"])
+        assert len(out["results"]) == 12 and all(r["same_result"] for r in out["results"])
         assert len(out["batch_results"]) == 3 and all(r["same_result"] for r in out["batch_results"])
         assert autotune.apply(tmp_path / "t.json") == out["best"]
         assert _lib.get_tuning()["walk"] == out["best"]["walk"]

@@ -72,7 +72,10 @@ def test_cell_beyond_max_cell_scan_large_scene(fx, oracle):
     holds the dense cell leaves the stream / row-batch / LDS-tile walk for the
     per-cell path that stops each cell's scan after 512 slots.  The lambda
     pass against the oracle (density bit-exact, lambda within the bar), and
-    two substeps bit-identical across the four walks (FxTuning.walk 0-3)."""
+    two substeps bit-identical across the five walks (FxTuning.walk 0-4:
+    walk 0 records lambda's pair masks for delta, and a particle whose row
+    holds the dense cell is not recorded, so its delta walks the full
+    stencil)."""
     import torch
 
     from flowx_amd import _lib
@@ -102,7 +105,7 @@ def test_cell_beyond_max_cell_scan_large_scene(fx, oracle):
     saved = _lib.get_tuning()
     res = []
     try:
-        for walk in (0, 1, 2, 3):
+        for walk in (0, 1, 2, 3, 4):
             _lib.set_tuning(walk=walk)
             s = fx.Solver(cfg, keep_scratch=True)
             s.reset(pos)
