@@ -21,7 +21,10 @@ from .config import seed_positions
 
 
 class BatchSolver:
-    def __init__(self, configs, device="cuda", fps=24.0):
+    def __init__(self, configs, device="cuda", fps=24.0, positions=None):
+        """positions: optional list of per-scene (N_i x 4 float32, w = 1)
+        starting positions (default: each scene's seeded lattice); scene i's
+        particle_count becomes N_i."""
         if not torch.cuda.is_available():
             raise RuntimeError("BatchSolver needs a ROCm GPU")
         _lib.lib()
@@ -29,7 +32,14 @@ class BatchSolver:
         self.device = torch.device(device)
         self.configs = list(configs)
         self.fps = fps
-        positions = [seed_positions(c) for c in self.configs]
+        if positions is None:
+            positions = [seed_positions(c) for c in self.configs]
+        else:
+            if len(positions) != len(self.configs):
+                raise ValueError("positions: one array per scene")
+            positions = [np.ascontiguousarray(np.asarray(p, np.float32).reshape(-1, 4)) for p in positions]
+            for c, p in zip(self.configs, positions):
+                c.particle_count = len(p)
         self.counts = [len(p) for p in positions]
         self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
         n = int(self.offsets[-1])

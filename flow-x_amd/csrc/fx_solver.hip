@@ -1058,11 +1058,16 @@ struct PairMasks {
 };
 constexpr int kMaskMaxRow = 64;
 
-// The row-batch walk of visit (VISIT_ROW_BATCH, row lengths packed two to a
-// register), recording each row's in-range bits: f returns whether the
-// candidate lies within h.  rm: this lane's first row word in LDS, `stride`
-// words between rows.  Returns false, having visited nothing, when a row is
-// longer than kMaskMaxRow or max_scan.
+// The row-batch walk of visit (VISIT_ROW_BATCH, all nine rows' bounds loaded
+// up front, row lengths packed two to a register), recording each row's
+// in-range bits: f returns whether the candidate lies within h.  rm: this
+// lane's first row word in LDS, `stride` words between rows.  Returns false,
+// having visited nothing, when a row is longer than kMaskMaxRow or max_scan
+// (the caller then walks the stencil without recording).  Loading each row's
+// bounds while the row before it is walked instead holds 8 registers fewer
+// but measured slower (lambda 335 -> 347-357 us at 1.64M, 4-6 loads per
+// batch; the batch's lambda 1215 -> 1192-1262 us); 5 loads per batch keeps
+// the plain pass at 8 waves per SIMD (6 loads: 65 registers).
 template <int B, class LD, class F>
 __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __restrict__ off, int max_scan,
                                              int bx, int by, int bz, LD&& ld, F&& f, uint64_t* rm, int stride) {
@@ -1122,7 +1127,10 @@ __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __res
 
 // Delta's walk over the recorded words: per row, B set bits at a time (the
 // batch's loads issued together, empty slots re-loading its first), the
-// candidates reaching f in slot order.
+// candidates reaching f in slot order.  Every row's word and first slot are
+// loaded up front (on the 1.64M scene and the 64-scene batch, against the
+// next row's loaded while the current one is walked: delta 155 -> 153 us,
+// batched 621 -> 594 us; 3 / 6 / 8 bits per batch: 156 / 159 / 165 us).
 template <int B, class LD, class F>
 __device__ __forceinline__ void visit_masked(const Grid& g, const int32_t* __restrict__ off, int bx, int by,
                                              int bz, int t, int n, const uint64_t* __restrict__ rows, LD&& ld,
@@ -1654,7 +1662,7 @@ __device__ __forceinline__ void lambda_body(int gt_in, int n, Grid g_, Consts k_
 #pragma unroll
           for (int r = 0; r < 9; ++r) pm.rows[(size_t)r * n + t] = rm[r * (int)blockDim.x];
         } else {
-          visit<VISIT_FULL>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
+          visit_auto<MODE | VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
         }
         pm.whole[t] = recorded ? 0 : 1;
       } else if constexpr ((V & kLdsWalk) != 0)
@@ -1792,7 +1800,7 @@ __device__ __forceinline__ void delta_body(int gt_in, int n, Grid g_, Consts k_,
           if (pm.whole[t] == 0)
             visit_masked<4>(g, off, bx, by, bz, t, n, pm.rows, ld, body);
           else
-            visit<VISIT_FULL>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
+            visit_auto<MODE | VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
         } else if constexpr ((V & kLdsWalk) != 0)
           visit_tile<6>(g, off, k.max_scan, bx, by, bz, *tile, ld, body);
         else
@@ -2354,6 +2362,17 @@ static int validate(const FxParams* p, const FxState* s, bool need_work) {
     } else                                                                                               \
       NLAUNCH(kernel<V>, n, st, __VA_ARGS__);                                                            \
   } while (0)
+// the same with `shm` bytes of dynamic LDS per workgroup
+#define BLAUNCH_SHM(kernel, V, n, shm, st, ...)                                                          \
+  do {                                                                                                   \
+    if (g_tuning.persistent > 0) {                                                                       \
+      FX_TRY(fill(S.queue, kXcds, 0, st));                                                               \
+      FX_LAUNCH_SHM(kernel<V + kPersistent>, dim3(persistent_grid(n, g_tuning.block)),                    \
+                    dim3(g_tuning.block), shm, st, __VA_ARGS__);                                         \
+    } else                                                                                               \
+      FX_LAUNCH_SHM(kernel<V>, dim3(grid_for(n, g_tuning.block)), dim3(g_tuning.block), shm, st,          \
+                    __VA_ARGS__);                                                                        \
+  } while (0)
 
 // Workgroups of a persistent launch over n threads: g_tuning.persistent per
 // CU, a multiple of the 8 XCDs, no more than the flat launch would use.
@@ -2895,13 +2914,25 @@ static int batch_step(const FxScene* scenes, int count, float* pos, float* vel, 
   FX_LAUNCH(k_mark_grid, dim3(1), dim3(64), st, w.flags, n, nc);
   if (load || n == 0) return check_launch("batch");
   int cur = cur0;
+  // the batch streams one lane per particle at any size, so the pair masks
+  // serve it whenever the tuning keeps the default walk
+  const bool masks = g_tuning.walk == 0;
+  const PairMasks pm = pair_masks(w);
   for (int it = 0; it < iters; ++it) {
-    BLAUNCH(k_lambda, 8 + VISIT_FULL, n, st, n, Grid{}, Consts{}, w.pred_s[cur], w.off, w.dl_s, w.sidx,
-            (float4*)nullptr, L0, S, it);
     // each scene writes its velocity on its own last iteration
-    BLAUNCH(k_delta, 8 + VISIT_FULL, n, st, n, Grid{}, Consts{}, (const float4*)w.pred_s[cur],
-            (const float2*)w.dl_s, w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s, w.vel_s, w.sidx,
-            (float4*)nullptr, L0, (const int32_t*)nullptr, S, it);
+    if (masks) {
+      BLAUNCH_SHM(k_lambda, 8 + VISIT_FULL + kMaskRecord, n, row_words_lds(g_tuning.block), st, n, Grid{}, Consts{},
+                  w.pred_s[cur], w.off, w.dl_s, w.sidx, (float4*)nullptr, L0, S, it, (const int32_t*)nullptr, pm);
+      BLAUNCH(k_delta, 8 + VISIT_FULL + kMaskWalk, n, st, n, Grid{}, Consts{}, (const float4*)w.pred_s[cur],
+              (const float2*)w.dl_s, w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s, w.vel_s, w.sidx,
+              (float4*)nullptr, L0, (const int32_t*)nullptr, S, it, (const int32_t*)nullptr, pm);
+    } else {
+      BLAUNCH(k_lambda, 8 + VISIT_FULL, n, st, n, Grid{}, Consts{}, w.pred_s[cur], w.off, w.dl_s, w.sidx,
+              (float4*)nullptr, L0, S, it);
+      BLAUNCH(k_delta, 8 + VISIT_FULL, n, st, n, Grid{}, Consts{}, (const float4*)w.pred_s[cur],
+              (const float2*)w.dl_s, w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s, w.vel_s, w.sidx,
+              (float4*)nullptr, L0, (const int32_t*)nullptr, S, it);
+    }
     cur ^= 1;
   }
   // no scene iterates: the batched delta launch as iteration 0 writes each

@@ -45,6 +45,43 @@ def test_batch_equals_plain_per_scene(fx):
         assert np.array_equal(batch.scene_velocities(i), s.read_velocities()), f"scene {i} velocities"
 
 
+@pytest.mark.parametrize("persistent", [0, 8])
+def test_batch_pair_masks_bit_identical(fx, persistent):
+    """The batch's lambda records each particle's in-range candidates per
+    stencil row and its delta walks only those (walk 0, flat and persistent
+    launches): equal bit for bit to the batch without them (walk 4).  One
+    scene holds a 120-particle clump in one cell, so some of its particles
+    have a row past the 64 slots a word records and their delta walks the
+    full stencil."""
+    import torch
+
+    from flowx_amd import _lib
+
+    rng = np.random.default_rng(3)
+    clump = np.concatenate([rng.uniform([0.021, 0.021, 0.121], [0.059, 0.059, 0.159], (120, 3)),
+                            np.ones((120, 1))], 1).astype(np.float32)
+    starts = [fx.seed_positions(c) for c in _scenes(fx)]
+    starts[2] = np.concatenate([starts[2], clump])
+
+    saved = _lib.get_tuning()
+    res = []
+    try:
+        for walk in (4, 0):
+            _lib.set_tuning(walk=walk, persistent=persistent)
+            batch = fx.BatchSolver(_scenes(fx), positions=starts)
+            for _ in range(4):
+                batch.substep()
+            torch.cuda.synchronize()
+            res.append([(batch.scene_positions(i), batch.scene_velocities(i)) for i in range(3)])
+            del batch
+    finally:
+        _lib.set_tuning(**saved)
+    for i in range(3):
+        assert np.isfinite(res[0][i][0]).all()
+        assert np.array_equal(res[0][i][0], res[1][i][0]), f"scene {i} positions"
+        assert np.array_equal(res[0][i][1], res[1][i][1]), f"scene {i} velocities"
+
+
 def test_batch_with_per_scene_iteration_counts(fx):
     """Scenes with 4, 2, 0, 6 and 3 Jacobi iterations (and their own dt) in
     one batch: the Jacobi launches run six times, a scene past its own count
