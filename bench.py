@@ -55,11 +55,19 @@ def algorithmic_bytes(kernel, n, c):
     return per_particle * n, per_particle
 
 
+def profile_name(kernel):
+    """File-name form of a kernel's launch label: 'k_lambda<VISIT_FULL + kMaskRecord>' ->
+    'k_lambda_VISIT_FULL_kMaskRecord'."""
+    import re
+
+    return re.sub(r"[^A-Za-z0-9]+", "_", kernel).strip("_")
+
+
 def read_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary
     (profiles/<kernel>_traffic.txt: 'fetch_bytes=' and 'write_bytes=' lines),
     or None."""
-    f = ROOT / "profiles" / f"{kernel.replace('<', '_').replace('>', '')}_traffic.txt"
+    f = ROOT / "profiles" / f"{profile_name(kernel)}_traffic.txt"
     if not f.exists():
         return None
     vals = {}

@@ -1131,6 +1131,11 @@ __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __res
 // loaded up front (on the 1.64M scene and the 64-scene batch, against the
 // next row's loaded while the current one is walked: delta 155 -> 153 us,
 // batched 621 -> 594 us; 3 / 6 / 8 bits per batch: 156 / 159 / 165 us).
+// Taking the three rows of a z-plane as one run, so a lane's batch runs on
+// across row ends instead of the wave waiting at each row's end for its lane
+// with the most bits, measured slower: 152 -> 155-166 us with 3-6 bits per
+// batch (batched 591 -> 643 us): the selects that pick each bit's row cost
+// more than the waits they save.
 template <int B, class LD, class F>
 __device__ __forceinline__ void visit_masked(const Grid& g, const int32_t* __restrict__ off, int bx, int by,
                                              int bz, int t, int n, const uint64_t* __restrict__ rows, LD&& ld,
