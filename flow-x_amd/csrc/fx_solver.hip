@@ -764,6 +764,11 @@ constexpr int VISIT_PACKED = 32;       // with VISIT_ROW_BATCH: row lengths two 
 constexpr int kQuadBelowLambda = 104000;
 constexpr int kQuadBelowDelta = 88000;
 constexpr int kQuadBelowXsph = 36000;
+// Delta over lambda's recorded pairs (pair masks): in quads below this, one
+// lane per particle above (h/2 lattices, cohesion on; delta us, quads / one
+// lane: 25,600 particles 21.2 / 25.5, 64,980 29.9 / 27.3, 89,780 37.1 / 28.5,
+// 127,008 46.7 / 29.6, 204,800 68.0 / 35.9).
+constexpr int kQuadMaskBelowDelta = 50000;
 // Kernel template bit: between its kQuadBelow* and kRowBelow, delta and
 // XSPH walk row by row, whose bookkeeping is lighter than the stream walks'
 // (25,600 particles: delta / XSPH 48 / 58 us row by row against 59 / 74 us
@@ -1460,6 +1465,150 @@ __device__ __forceinline__ void visit_quad(const Grid& g, const int32_t* __restr
   }
 }
 
+// Pair masks for the quad walk (small scenes).  Recording: the quad's lanes
+// walk each row as visit_quad does; lane j's in-range bits (its slots
+// k + 4u + j) gather in its own word, the quad ORs the four words at the row's
+// end (DPP broadcasts, every lane active), and lane 0 parks the row's word in
+// LDS at rm[r * stride].  tm(q, data, valid, in) also sets `in` (within h).
+// Returns whether every row was recorded (none past kMaskMaxRow or max_scan;
+// such a row is walked cell by cell or whole without recording).
+template <int B, class LD, class TM, class FD>
+__device__ __forceinline__ bool visit_quad_record(const Grid& g, const int32_t* __restrict__ off, int max_scan,
+                                                  int bx, int by, int bz, int j, LD&& ld, TM&& tm, FD&& fold,
+                                                  uint64_t* rm, int stride) {
+  const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
+  auto row = [&](int r, int& c0, int& s, int& e) {
+    const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
+    if (z < 0 || z >= g.cz || y < 0 || y >= g.cy) {
+      c0 = -1;
+      s = e = 0;
+      return;
+    }
+    c0 = (z * g.cy + y) * g.cx + x0;
+    s = off[c0];
+    e = off[c0 + (x1 - x0) + 1];
+  };
+  auto plain_tm = [&](int q, const auto& d, bool valid) {
+    bool in;
+    return tm(q, d, valid, in);
+  };
+  bool whole = true;
+  int c0n, sn, en;
+  row(0, c0n, sn, en);
+#pragma unroll 1
+  for (int r = 0; r < 9; ++r) {
+    const int c0 = c0n, s = sn, e = en;
+    if (r < 8) row(r + 1, c0n, sn, en);
+    uint64_t acc = 0;
+    if (c0 >= 0) {
+      if (e - s > max_scan) {
+        whole = false;
+        for (int c = c0; c <= c0 + (x1 - x0); ++c) {
+          const int cs = off[c], ce = min(off[c + 1], cs + max_scan);
+          quad_run<B>(cs, ce, j, ld, plain_tm, fold);
+        }
+      } else if (e - s > kMaskMaxRow) {
+        whole = false;
+        quad_run<B>(s, e, j, ld, plain_tm, fold);
+      } else {
+#pragma unroll 1
+        for (int k = s; k < e; k += 4 * B) {
+          decltype(ld(0)) d[B];
+#pragma unroll
+          for (int u = 0; u < B; ++u) d[u] = ld(min(k + 4 * u + j, e - 1));
+          decltype(plain_tm(0, d[0], false)) tt[B];
+#pragma unroll
+          for (int u = 0; u < B; ++u) {
+            const int q = k + 4 * u + j;
+            bool in;
+            tt[u] = tm(q, d[u], q < e, in);
+            if (in) acc |= 1ull << (q - s);
+          }
+#pragma unroll
+          for (int u = 0; u < B; ++u) {
+            if (k + 4 * u < e) {  // the same for the four lanes
+              fold(tt[u].template lane<0>());
+              fold(tt[u].template lane<1>());
+              fold(tt[u].template lane<2>());
+              fold(tt[u].template lane<3>());
+            }
+          }
+        }
+      }
+    }
+    uint32_t lo = (uint32_t)acc, hi = (uint32_t)(acc >> 32);
+    lo = __builtin_bit_cast(uint32_t, quad_lane<0>(__builtin_bit_cast(float, lo))) |
+         __builtin_bit_cast(uint32_t, quad_lane<1>(__builtin_bit_cast(float, lo))) |
+         __builtin_bit_cast(uint32_t, quad_lane<2>(__builtin_bit_cast(float, lo))) |
+         __builtin_bit_cast(uint32_t, quad_lane<3>(__builtin_bit_cast(float, lo)));
+    hi = __builtin_bit_cast(uint32_t, quad_lane<0>(__builtin_bit_cast(float, hi))) |
+         __builtin_bit_cast(uint32_t, quad_lane<1>(__builtin_bit_cast(float, hi))) |
+         __builtin_bit_cast(uint32_t, quad_lane<2>(__builtin_bit_cast(float, hi))) |
+         __builtin_bit_cast(uint32_t, quad_lane<3>(__builtin_bit_cast(float, hi)));
+    if (j == 0) rm[r * stride] = ((uint64_t)hi << 32) | lo;
+  }
+  return whole;
+}
+
+// Delta's quad walk over the recorded words: in each round the quad's lanes
+// take the next four set bits of the row (lane j the j-th), so the quad folds
+// the terms in slot order as visit_quad does; B rounds of loads in flight.  A
+// lane past the row's last bit loads the row's first slot and its term is the
+// zero term (tm sees valid = false).
+template <int B, class LD, class TM, class FD>
+__device__ __forceinline__ void visit_quad_masked(const Grid& g, const int32_t* __restrict__ off, int bx, int by,
+                                                  int bz, int j, int t, int n, const uint64_t* __restrict__ rows,
+                                                  LD&& ld, TM&& tm, FD&& fold) {
+  const int x0 = max(bx - 1, 0);
+  auto row = [&](int r, int& s, uint64_t& m) {
+    const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
+    const bool in = z >= 0 && z < g.cz && y >= 0 && y < g.cy;
+    s = in ? off[(z * g.cy + y) * g.cx + x0] : 0;
+    m = rows[(size_t)r * n + t];
+  };
+  int sn;
+  uint64_t mn;
+  row(0, sn, mn);
+#pragma unroll 1
+  for (int r = 0; r < 9; ++r) {
+    const int s = sn;
+    uint64_t m = mn;
+    if (r < 8) row(r + 1, sn, mn);
+#pragma unroll 1
+    while (m) {  // the same for the four lanes
+      int q[B];
+      bool v[B], any[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        uint64_t mm = m;
+        if (j >= 1) mm &= mm - 1;
+        if (j >= 2) mm &= mm - 1;
+        if (j >= 3) mm &= mm - 1;
+        any[u] = m != 0;
+        v[u] = mm != 0;
+        q[u] = v[u] ? s + __ffsll((unsigned long long)mm) - 1 : s;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m &= m - 1;
+      }
+      decltype(ld(0)) d[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) d[u] = ld(q[u]);
+      decltype(tm(0, d[0], false)) tt[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) tt[u] = tm(q[u], d[u], v[u]);
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        if (any[u]) {
+          fold(tt[u].template lane<0>());
+          fold(tt[u].template lane<1>());
+          fold(tt[u].template lane<2>());
+          fold(tt[u].template lane<3>());
+        }
+      }
+    }
+  }
+}
+
 // Lambda's per-candidate terms and their fold, shared by every walk: the
 // density term m ((coef d) d) d (the reference's operations, so the density
 // sum stays bit-exact), and the gradient c d, c = spiky (h - r)^2 / r without
@@ -1636,7 +1785,27 @@ __device__ __forceinline__ void lambda_body(int gt_in, int n, Grid g_, Consts k_
         const bool in = valid && r2 < h2, grad = in && r2 > 1e-12f;
         return lam_grad(in ? lam_dens(k, r2) : 0.0f, grad ? lam_coef(k, r2) : 0.0f, dx, dy, dz);
       };
-      visit_quad<8>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const LamTerm& tt) { sum.add(tt); });
+      if constexpr (REC) {
+        auto tm_rec = [&](int q, const float4& pj, bool valid, bool& in_h) {
+          const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+          const float r2 = dist2_exact(dx, dy, dz);
+          const bool in = valid && r2 < h2, grad = in && r2 > 1e-12f;
+          in_h = in;
+          return lam_grad(in ? lam_dens(k, r2) : 0.0f, grad ? lam_coef(k, r2) : 0.0f, dx, dy, dz);
+        };
+        const int stride = (int)blockDim.x >> 2;
+        const bool recorded = visit_quad_record<8>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm_rec,
+                                                   [&](const LamTerm& tt) { sum.add(tt); }, rm, stride);
+        if ((gt & 3) == 0) {
+          if (recorded) {
+#pragma unroll
+            for (int r = 0; r < 9; ++r) pm.rows[(size_t)r * n + t] = rm[r * stride];
+          }
+          pm.whole[t] = recorded ? 0 : 1;
+        }
+      } else {
+        visit_quad<8>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const LamTerm& tt) { sum.add(tt); });
+      }
       if (gt & 3) return;
     } else {
       auto body = [&](int q, const float4& pj) -> bool {
@@ -1701,10 +1870,11 @@ __global__ __launch_bounds__(256) void k_lambda(int n, Grid g_, Consts k_,
                                                 int iter = -1, const int32_t* __restrict__ tkey = nullptr,
                                                 PairMasks pm = PairMasks{nullptr, nullptr}) {
   if constexpr ((V & kMaskRecord) != 0) {
-    // each lane's nine row words in LDS, row r at rm[r * blockDim.x]
-    // (launched with 9 * 8 * blockDim.x bytes of dynamic LDS)
+    // each lane's (the quad walk: each quad's) nine row words in LDS, row r
+    // at rm[r * blockDim.x] (rm[r * blockDim.x / 4]); launched with
+    // row_words_lds bytes of dynamic LDS
     extern __shared__ uint64_t fx_row_words[];
-    uint64_t* rm = fx_row_words + threadIdx.x;
+    uint64_t* rm = fx_row_words + (((V & kQuadWalk) != 0) ? threadIdx.x >> 2 : threadIdx.x);
     if constexpr ((V & kPersistent) != 0) {
       constexpr bool QUADW = (V & kQuadWalk) != 0;
       const long long nth = (QUADW ? 4LL * n : (long long)n);
@@ -1791,7 +1961,15 @@ __device__ __forceinline__ void delta_body(int gt_in, int n, Grid g_, Consts k_,
           const bool in = valid && r2 < h2 && r2 > 1e-12f;  // q == t gives r2 = +0
           return DelTerm{in ? del_coef(k, li, pj.w, inv_wq, scorr, r2) : 0.0f, dx, dy, dz};
         };
-        visit_quad<4>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const DelTerm& tt) { sum.add(tt); });
+        if constexpr ((V & kMaskWalk) != 0) {
+          if (pm.whole[t] == 0)
+            visit_quad_masked<4>(g, off, bx, by, bz, gt & 3, t, n, pm.rows, ld, tm,
+                                 [&](const DelTerm& tt) { sum.add(tt); });
+          else
+            visit_quad<4>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const DelTerm& tt) { sum.add(tt); });
+        } else {
+          visit_quad<4>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const DelTerm& tt) { sum.add(tt); });
+        }
         if (gt & 3) return;
       } else {
         auto body = [&](int q, const float4& pj) -> bool {
@@ -2413,12 +2591,16 @@ static Lists make_lists(const Ws& w, const Consts& k, bool enabled) {
   return L;
 }
 
-// Pair masks (PairMasks) serve the large-scene walks without neighbour lists,
-// unless the tuning asks for the walks without them (walk 4).  Their words
-// take the lists' mask area and whole-stencil flags.
-static bool pair_masks_on(long long n, bool lists) { return !lists && g_tuning.walk == 0 && n >= kRowBelow; }
+// Pair masks (PairMasks) serve every walk without neighbour lists (the quad
+// walk of small scenes records and walks them in quads), unless the tuning
+// asks for the walks without them (walk 4).  Their words take the lists' mask
+// area and whole-stencil flags.
+static bool pair_masks_on(bool lists) { return !lists && g_tuning.walk == 0; }
 static PairMasks pair_masks(const Ws& w) { return PairMasks{w.mask, w.lfull}; }
-static size_t row_words_lds(int block) { return (size_t)9 * sizeof(uint64_t) * (size_t)block; }
+// nine words per lane, or per quad of lanes in the quad walk
+static size_t row_words_lds(int block, bool quad = false) {
+  return (size_t)9 * sizeof(uint64_t) * (size_t)(quad ? block / 4 : block);
+}
 
 static int fill(int32_t* a, long long n, int32_t v, hipStream_t st) {
   if (n <= 0) return FX_OK;
@@ -2574,12 +2756,16 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
     FX_TRY(fill(w.dcell, p.n_cells, 0, st));
     FX_TRY(fill(w.flags + 2, 1, 0, st));
   }
-  const bool masks = pair_masks_on(n, lists);
+  const bool masks = pair_masks_on(lists);
   const PairMasks pm = pair_masks(w);
   for (int it = 0; it < iters; ++it) {
     const bool last = it == iters - 1;
     float4* lam_out = (last && last_in_call && s.lambda) ? (float4*)s.lambda : (float4*)nullptr;
-    if (masks)
+    if (masks && n < kQuadBelowLambda)
+      FX_LAUNCH_SHM(k_lambda<VISIT_FULL + kQuadWalk + kMaskRecord>, dim3(grid_for(4LL * n, g_tuning.block)),
+                    dim3(g_tuning.block), row_words_lds(g_tuning.block, true), st, n, g, k, w.pred_s[cur], w.off,
+                    w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0}, -1, (const int32_t*)nullptr, pm);
+    else if (masks)
       FX_LAUNCH_SHM(k_lambda<VISIT_FULL + kMaskRecord>, dim3(grid_for(n, g_tuning.block)), dim3(g_tuning.block),
                     row_words_lds(g_tuning.block), st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L,
                     Scenes{nullptr, 0}, -1, (const int32_t*)nullptr, pm);
@@ -2592,7 +2778,12 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
       NLAUNCH(k_lambda<VISIT_BUILD>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
     else
       NLAUNCH(k_lambda<VISIT_LIST>, n, st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0});
-    if (masks)
+    if (masks && n < kQuadMaskBelowDelta)
+      FX_LAUNCH(k_delta<VISIT_FULL + kQuadWalk + kMaskWalk>, dim3(grid_for(4LL * n, g_tuning.block)),
+                dim3(g_tuning.block), st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
+                w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
+                (float4*)nullptr, L, (const int32_t*)nullptr, Scenes{nullptr, 0}, -1, (const int32_t*)nullptr, pm);
+    else if (masks)
       NLAUNCH(k_delta<VISIT_FULL + kMaskWalk>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s,
               w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
               (float4*)nullptr, L, (const int32_t*)nullptr, Scenes{nullptr, 0}, -1, (const int32_t*)nullptr, pm);

@@ -189,3 +189,43 @@ def test_single_particle_free_fall_and_empty_scene(fx, oracle):
     g0 = GpuState(fx, p0)
     g0.run("load_state")
     g0.run("substep")  # nothing to do, and no error
+
+
+@pytest.mark.parametrize("lattice", [(40, 40, 16), (70, 68, 20)])
+def test_small_scene_pair_masks_bit_identical(fx, lattice):
+    """Small and mid-size scenes (25,600 and 95,200 lattice particles: lambda
+    records in quads, delta walks the recorded words in quads at the smaller
+    size and with one lane per particle at the larger) plus a 150-particle clump
+    in one cell, whose rows pass the 64 slots a word records: two substeps
+    with the pair masks (walk 0) equal the walks without them (walk 4) bit
+    for bit."""
+    import torch
+
+    from flowx_amd import _lib
+    from flowx_amd.config import converging_update
+
+    rng = np.random.default_rng(21)
+    cfg = converging_update(fx.lattice_config((-1.0, -1.0, 0.0), (1.0, 1.0, 2.0), 0.025, lattice, iterations=4,
+                                              viscosity=0.01, surface_tension=0.001))
+    cfg.extra.update(colliders=[("box", (-0.2, -0.2, 0.0), (0.2, 0.2, 0.3))], collider_resolution=40)
+    clump = np.concatenate([rng.uniform([0.0026, 0.0026, 0.1026], [0.0224, 0.0224, 0.1224], (150, 3)),
+                            np.ones((150, 1))], 1).astype(np.float32)
+    pos = np.concatenate([fx.seed_positions(cfg), clump])
+    cfg.particle_count = len(pos)
+    _, dt = cfg.substep_dt(1.0 / 24.0)
+    saved = _lib.get_tuning()
+    res = []
+    try:
+        for walk in (4, 0):
+            _lib.set_tuning(walk=walk)
+            s = fx.Solver(cfg, keep_scratch=True)
+            s.reset(pos)
+            s.substeps_n(dt, 2)
+            torch.cuda.synchronize()
+            res.append((s.pos.cpu().numpy(), s.vel.cpu().numpy(), s.lam.cpu().numpy()))
+            del s
+    finally:
+        _lib.set_tuning(**saved)
+    assert np.isfinite(res[0][0]).all()
+    for name, a, b in zip(("pos", "vel", "lam"), res[0], res[1]):
+        assert np.array_equal(a, b), name
