@@ -1067,15 +1067,16 @@ constexpr int kMaskMaxRow = 64;
 // up front, row lengths packed two to a register), recording each row's
 // in-range bits: f returns whether the candidate lies within h.  rm: this
 // lane's first row word in LDS, `stride` words between rows.  Returns false,
-// having visited nothing, when a row is longer than kMaskMaxRow or max_scan
-// (the caller then walks the stencil without recording).  Loading each row's
-// bounds while the row before it is walked instead holds 8 registers fewer
-// but measured slower (lambda 335 -> 347-357 us at 1.64M, 4-6 loads per
-// batch; the batch's lambda 1215 -> 1192-1262 us); 5 loads per batch keeps
-// the plain pass at 8 waves per SIMD (6 loads: 65 registers).
+// having visited nothing, when a row is longer than kMaskMaxRow or max_scan;
+// the caller then walks visit_cells_capped.  (Walking such rows here, or
+// falling back to the stream walk, holds 9-13 more registers in the batched
+// pass: 6 waves per SIMD instead of 7.)  Loading each row's bounds while the
+// row before it is walked instead holds 8 registers fewer but measured slower
+// (lambda 335 -> 347-357 us at 1.64M, 4-6 loads per batch; the batch's
+// lambda 1215 -> 1192-1262 us).
 template <int B, class LD, class F>
 __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __restrict__ off, int max_scan,
-                                             int bx, int by, int bz, LD&& ld, F&& f, uint64_t* rm, int stride) {
+                                            int bx, int by, int bz, LD&& ld, F&& f, uint64_t* rm, int stride) {
   static_assert(B <= kCandPad, "row batches read up to B - 1 slots past a run");
   const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
   int S[9];
@@ -1095,7 +1096,7 @@ __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __res
       Lp[r / 2] |= (uint32_t)len << 16;
   }
   if (big) return false;
-  int k = S[0], e = S[0] + (int)(Lp[0] & 0xFFFFu), left = 8;
+  int k = S[0], e = S[0] + (int)(Lp[0] & 0xFFFFu), left = 8;  // the current row is 8 - left
   uint64_t acc = 0;
   auto next_row = [&]() {
     while (k >= e && left > 0) {
@@ -1128,6 +1129,26 @@ __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __res
   }
   rm[(8 - left) * stride] = acc;
   return true;
+}
+
+// The 27 stencil cells one at a time, each capped at max_scan: the
+// reference's order with few registers, for the rare particle with a row the
+// pair masks do not record.
+template <class LD, class F>
+__device__ __forceinline__ void visit_cells_capped(const Grid& g, const int32_t* __restrict__ off, int max_scan,
+                                                   int bx, int by, int bz, LD&& ld, F&& f) {
+  const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
+#pragma unroll 1
+  for (int r = 0; r < 9; ++r) {
+    const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
+    if (z < 0 || z >= g.cz || y < 0 || y >= g.cy) continue;
+    const int c0 = (z * g.cy + y) * g.cx + x0;
+#pragma unroll 1
+    for (int c = c0; c <= c0 + (x1 - x0); ++c) {
+      const int cs = off[c], ce = min(off[c + 1], cs + max_scan);
+      scan_run<4>(cs, ce, ld, f);
+    }
+  }
 }
 
 // Delta's walk over the recorded words: per row, B set bits at a time (the
@@ -1831,12 +1852,14 @@ __device__ __forceinline__ void lambda_body(int gt_in, int n, Grid g_, Consts k_
           }
           return r2 < h2;
         };
-        const bool recorded = visit_record<5>(g, off, k.max_scan, bx, by, bz, ld, body_rec, rm, (int)blockDim.x);
+        // (4 loads per batch in the batched pass: 73 -> 69 registers, 7 waves per SIMD)
+        const bool recorded =
+            visit_record<(V & 8) ? 4 : 5>(g, off, k.max_scan, bx, by, bz, ld, body_rec, rm, (int)blockDim.x);
         if (recorded) {
 #pragma unroll
           for (int r = 0; r < 9; ++r) pm.rows[(size_t)r * n + t] = rm[r * (int)blockDim.x];
         } else {
-          visit_auto<MODE | VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
+          visit_cells_capped(g, off, k.max_scan, bx, by, bz, ld, body_rec);
         }
         pm.whole[t] = recorded ? 0 : 1;
       } else if constexpr ((V & kLdsWalk) != 0)
