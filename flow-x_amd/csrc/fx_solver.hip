@@ -1151,17 +1151,17 @@ __device__ __forceinline__ void visit_cells_capped(const Grid& g, const int32_t*
   }
 }
 
-// Delta's walk over the recorded words: per row, B set bits at a time (the
-// batch's loads issued together, empty slots re-loading its first), the
-// candidates reaching f in slot order.  Every row's word and first slot are
-// loaded up front (on the 1.64M scene and the 64-scene batch, against the
-// next row's loaded while the current one is walked: delta 155 -> 153 us,
-// batched 621 -> 594 us; 3 / 6 / 8 bits per batch: 156 / 159 / 165 us).
-// Taking the three rows of a z-plane as one run, so a lane's batch runs on
-// across row ends instead of the wave waiting at each row's end for its lane
-// with the most bits, measured slower: 152 -> 155-166 us with 3-6 bits per
-// batch (batched 591 -> 643 us): the selects that pick each bit's row cost
-// more than the waits they save.
+// Delta's walk over the recorded words: B set bits of the current row per
+// batch (the batch's loads issued together, empty slots re-loading its
+// first), the candidates reaching f in slot order, each lane running on to
+// its next row without waiting for its wave at row ends (the nine words and
+// first slots shift down a queue as the lane leaves a row, which keeps them
+// in registers).  On the 1.64M scene (delta us): 134 with 4 bits per batch,
+// 141 / 137 with 3 / 6; walked row by row, the wave waiting at each row's
+// end for its lane with the most bits, 152 (3 / 6 / 8 bits: 156 / 159 /
+// 165), and with only the next row's word loaded ahead 155; a z-plane of
+// three rows taken as one run, a batch's bits picked across the plane's rows
+// by selects, 155-166.  The 64-scene batch: 565 against 592 row by row.
 template <int B, class LD, class F>
 __device__ __forceinline__ void visit_masked(const Grid& g, const int32_t* __restrict__ off, int bx, int by,
                                              int bz, int t, int n, const uint64_t* __restrict__ rows, LD&& ld,
@@ -1176,27 +1176,36 @@ __device__ __forceinline__ void visit_masked(const Grid& g, const int32_t* __res
     S[r] = in ? off[(z * g.cy + y) * g.cx + x0] : 0;
     M[r] = rows[(size_t)r * n + t];
   }
+  int left = 8;
+  auto next_row = [&]() {
+    while (M[0] == 0 && left > 0) {
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    uint64_t m = M[r];
-    const int s = S[r];
-#pragma unroll 1
-    while (m) {
-      int q[B];
-      bool v[B];
-#pragma unroll
-      for (int u = 0; u < B; ++u) {
-        v[u] = m != 0;
-        q[u] = s + __ffsll((unsigned long long)m) - 1;  // (s - 1 once m is empty: not loaded)
-        m &= m - 1;
+      for (int r = 0; r < 8; ++r) {
+        M[r] = M[r + 1];
+        S[r] = S[r + 1];
       }
-      decltype(ld(0)) d[B];
-#pragma unroll
-      for (int u = 0; u < B; ++u) d[u] = ld(v[u] ? q[u] : q[0]);
-#pragma unroll
-      for (int u = 0; u < B; ++u)
-        if (v[u]) f(q[u], d[u]);
+      M[8] = 0;
+      --left;
     }
+  };
+  next_row();
+#pragma unroll 1
+  while (M[0]) {
+    int q[B];
+    bool v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      v[u] = M[0] != 0;
+      q[u] = S[0] + __ffsll((unsigned long long)M[0]) - 1;  // (S[0] - 1 once the row is empty: not loaded)
+      M[0] &= M[0] - 1;
+    }
+    decltype(ld(0)) d[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) d[u] = ld(v[u] ? q[u] : q[0]);
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (v[u]) f(q[u], d[u]);
+    next_row();
   }
 }
 
