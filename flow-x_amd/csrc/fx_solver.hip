@@ -766,9 +766,9 @@ constexpr int kQuadBelowDelta = 88000;
 constexpr int kQuadBelowXsph = 36000;
 // Delta over lambda's recorded pairs (pair masks): in quads below this, one
 // lane per particle above (h/2 lattices, cohesion on; delta us, quads / one
-// lane: 25,600 particles 21.2 / 25.5, 64,980 29.9 / 27.3, 89,780 37.1 / 28.5,
-// 127,008 46.7 / 29.6, 204,800 68.0 / 35.9).
-constexpr int kQuadMaskBelowDelta = 50000;
+// lane: 25,600 particles 18.1 / 21.1, 40,000 21.6 / 21.3, 50,000 23.7 /
+// 21.5, 64,980 25.6 / 21.4, 89,780 31.7 / 23.2).
+constexpr int kQuadMaskBelowDelta = 40000;
 // Kernel template bit: between its kQuadBelow* and kRowBelow, delta and
 // XSPH walk row by row, whose bookkeeping is lighter than the stream walks'
 // (25,600 particles: delta / XSPH 48 / 58 us row by row against 59 / 74 us
@@ -1581,61 +1581,73 @@ __device__ __forceinline__ bool visit_quad_record(const Grid& g, const int32_t* 
 }
 
 // Delta's quad walk over the recorded words: in each round the quad's lanes
-// take the next four set bits of the row (lane j the j-th), so the quad folds
-// the terms in slot order as visit_quad does; B rounds of loads in flight.  A
-// lane past the row's last bit loads the row's first slot and its term is the
-// zero term (tm sees valid = false).
+// take the next four set bits of the current row (lane j the j-th), so the
+// quad folds the terms in slot order as visit_quad does; B rounds of loads in
+// flight, all from one row.  Each quad runs on to its next row without
+// waiting for the rest of its wave (the nine words and first slots shift down
+// a queue, as in visit_masked).  A lane past the row's last bit loads the
+// row's first slot and its term is the zero term (tm sees valid = false).
+// Dam break (25,600 particles), delta us: 17.5 / 17.7 / 20.8 with 2 / 1 / 4
+// rounds per batch; row by row with the wave in step (4 rounds) 20.3.
 template <int B, class LD, class TM, class FD>
 __device__ __forceinline__ void visit_quad_masked(const Grid& g, const int32_t* __restrict__ off, int bx, int by,
                                                   int bz, int j, int t, int n, const uint64_t* __restrict__ rows,
                                                   LD&& ld, TM&& tm, FD&& fold) {
   const int x0 = max(bx - 1, 0);
-  auto row = [&](int r, int& s, uint64_t& m) {
+  int S[9];
+  uint64_t M[9];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
     const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
     const bool in = z >= 0 && z < g.cz && y >= 0 && y < g.cy;
-    s = in ? off[(z * g.cy + y) * g.cx + x0] : 0;
-    m = rows[(size_t)r * n + t];
-  };
-  int sn;
-  uint64_t mn;
-  row(0, sn, mn);
-#pragma unroll 1
-  for (int r = 0; r < 9; ++r) {
-    const int s = sn;
-    uint64_t m = mn;
-    if (r < 8) row(r + 1, sn, mn);
-#pragma unroll 1
-    while (m) {  // the same for the four lanes
-      int q[B];
-      bool v[B], any[B];
+    S[r] = in ? off[(z * g.cy + y) * g.cx + x0] : 0;
+    M[r] = rows[(size_t)r * n + t];
+  }
+  int left = 8;
+  auto next_row = [&]() {
+    while (M[0] == 0 && left > 0) {  // the same for the four lanes
 #pragma unroll
-      for (int u = 0; u < B; ++u) {
-        uint64_t mm = m;
-        if (j >= 1) mm &= mm - 1;
-        if (j >= 2) mm &= mm - 1;
-        if (j >= 3) mm &= mm - 1;
-        any[u] = m != 0;
-        v[u] = mm != 0;
-        q[u] = v[u] ? s + __ffsll((unsigned long long)mm) - 1 : s;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) m &= m - 1;
+      for (int r = 0; r < 8; ++r) {
+        M[r] = M[r + 1];
+        S[r] = S[r + 1];
       }
-      decltype(ld(0)) d[B];
+      M[8] = 0;
+      --left;
+    }
+  };
+  next_row();
+#pragma unroll 1
+  while (M[0]) {  // the same for the four lanes
+    int q[B];
+    bool v[B], any[B];
 #pragma unroll
-      for (int u = 0; u < B; ++u) d[u] = ld(q[u]);
-      decltype(tm(0, d[0], false)) tt[B];
+    for (int u = 0; u < B; ++u) {
+      uint64_t mm = M[0];
+      if (j >= 1) mm &= mm - 1;
+      if (j >= 2) mm &= mm - 1;
+      if (j >= 3) mm &= mm - 1;
+      any[u] = M[0] != 0;
+      v[u] = mm != 0;
+      q[u] = v[u] ? S[0] + __ffsll((unsigned long long)mm) - 1 : S[0];
 #pragma unroll
-      for (int u = 0; u < B; ++u) tt[u] = tm(q[u], d[u], v[u]);
+      for (int i = 0; i < 4; ++i) M[0] &= M[0] - 1;
+    }
+    decltype(ld(0)) d[B];
 #pragma unroll
-      for (int u = 0; u < B; ++u) {
-        if (any[u]) {
-          fold(tt[u].template lane<0>());
-          fold(tt[u].template lane<1>());
-          fold(tt[u].template lane<2>());
-          fold(tt[u].template lane<3>());
-        }
+    for (int u = 0; u < B; ++u) d[u] = ld(q[u]);
+    decltype(tm(0, d[0], false)) tt[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) tt[u] = tm(q[u], d[u], v[u]);
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      if (any[u]) {
+        fold(tt[u].template lane<0>());
+        fold(tt[u].template lane<1>());
+        fold(tt[u].template lane<2>());
+        fold(tt[u].template lane<3>());
       }
     }
+    next_row();
   }
 }
 
@@ -1996,7 +2008,7 @@ __device__ __forceinline__ void delta_body(int gt_in, int n, Grid g_, Consts k_,
         };
         if constexpr ((V & kMaskWalk) != 0) {
           if (pm.whole[t] == 0)
-            visit_quad_masked<4>(g, off, bx, by, bz, gt & 3, t, n, pm.rows, ld, tm,
+            visit_quad_masked<2>(g, off, bx, by, bz, gt & 3, t, n, pm.rows, ld, tm,
                                  [&](const DelTerm& tt) { sum.add(tt); });
           else
             visit_quad<4>(g, off, k.max_scan, bx, by, bz, gt & 3, ld, tm, [&](const DelTerm& tt) { sum.add(tt); });
