@@ -329,6 +329,9 @@ struct Workspace {
   // views into the neighbour-list masks (unused while the lists are off):
   // the state carried in sorted order between the substeps of one call
   size_t pair64, pos_r, vel_r;
+  // candidate positions as three float arrays (x, y, z), read by lambda's
+  // recording walk four slots to a load
+  size_t sx, sy, sz;
   size_t total;
 };
 

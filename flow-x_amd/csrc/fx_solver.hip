@@ -167,6 +167,11 @@ Workspace workspace_layout(const FxParams& p) {
   w.flags = o;    o += align256(64 * 4);
   w.normal_next = o; o += align256(n * 16 + 16);
   w.sdcell = o;   o += align256(c * 4 + 4);
+  // x, y, z of the candidate positions, each with 16 floats of padding (a
+  // recording batch reads up to 7 slots past a run)
+  w.sx = o;       o += align256(n * 4 + 64);
+  w.sy = o;       o += align256(n * 4 + 64);
+  w.sz = o;       o += align256(n * 4 + 64);
   // 8 + 16 + 16 bytes per particle inside the 72 of the masks
   w.pair64 = w.mask;
   w.pos_r = w.pair64 + align256(n * 8);
@@ -186,6 +191,7 @@ struct Ws {
   int32_t *lcell, *lfull;
   float* disp;
   int32_t* dcell;
+  float *sx, *sy, *sz;
 };
 
 static Ws bind_ws(const Workspace& w, void* base) {
@@ -213,6 +219,9 @@ static Ws bind_ws(const Workspace& w, void* base) {
   s.lfull = (int32_t*)(b + w.lfull);
   s.disp = (float*)(b + w.disp);
   s.dcell = (int32_t*)(b + w.dcell);
+  s.sx = (float*)(b + w.sx);
+  s.sy = (float*)(b + w.sy);
+  s.sz = (float*)(b + w.sz);
   return s;
 }
 
@@ -508,6 +517,20 @@ __global__ void k_mark_grid(int32_t* __restrict__ flags, int n, int nc) {
   flags[kHdrError] = 0;
 }
 
+// The candidate positions as three float arrays (x, y, z by slot), written
+// beside pred_s by the passes that write it when the next lambda pass reads
+// its candidates four slots to a load (PairMasks); null x: not written.
+struct SoaPos {
+  float *x, *y, *z;
+};
+__device__ __forceinline__ void soa_store(const SoaPos& a, int t, float x, float y, float z) {
+  if (a.x) {
+    a.x[t] = x;
+    a.y[t] = y;
+    a.z[t] = z;
+  }
+}
+
 // The gather of a resident build: slot u of the new order reads its previous
 // slot (mostly a nearby one: particles rarely change cells), and records the
 // particle in sidx (plus the caller's reference-format copies when given).
@@ -516,7 +539,8 @@ __global__ void k_gather_pairs(int n, const uint64_t* __restrict__ pairs, const 
                                float4* __restrict__ pred_s, float4* __restrict__ pos_s,
                                float2* __restrict__ dl_s, int32_t* __restrict__ skey_out,
                                int32_t* __restrict__ skey_int, int32_t* __restrict__ sidx,
-                               int32_t* __restrict__ sidx_out, int32_t* __restrict__ key_canon) {
+                               int32_t* __restrict__ sidx_out, int32_t* __restrict__ key_canon,
+                               SoaPos soa = SoaPos{nullptr, nullptr, nullptr}) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= n) return;
   const uint64_t e = pairs[u];
@@ -525,6 +549,7 @@ __global__ void k_gather_pairs(int n, const uint64_t* __restrict__ pairs, const 
   float4 q = pred_r[src];
   q.w = 0.0f;
   pred_s[u] = q;
+  soa_store(soa, u, q.x, q.y, q.z);
   pos_s[u] = pos_r[src];
   dl_s[u] = make_float2(0.0f, 0.0f);
   const int k = key_r[src];
@@ -633,7 +658,7 @@ __global__ void k_gather(int n, const int32_t* __restrict__ sidx, const int32_t*
                          const float4* __restrict__ lambda4, float4* __restrict__ pred_s,
                          float4* __restrict__ pos_s, float2* __restrict__ dl_s,
                          int32_t* __restrict__ skey_out, int32_t* __restrict__ skey_int,
-                         int32_t* __restrict__ sidx_out) {
+                         int32_t* __restrict__ sidx_out, SoaPos soa = SoaPos{nullptr, nullptr, nullptr}) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = min(max(sidx[t], 0), n - 1);  // a caller-supplied permutation stays in bounds
@@ -645,6 +670,7 @@ __global__ void k_gather(int n, const int32_t* __restrict__ sidx, const int32_t*
   }
   q.w = dl.x;
   pred_s[t] = q;
+  soa_store(soa, t, q.x, q.y, q.z);
   if (pos_s) pos_s[t] = pos[i];
   if (dl_s) dl_s[t] = dl;
   if (key) {
@@ -1060,6 +1086,8 @@ __device__ __forceinline__ void visit(const Grid& g, const int32_t* __restrict__
 struct PairMasks {
   uint64_t* rows;  // [9][n]
   int32_t* whole;  // [n]: 1 = not recorded, walk the full stencil
+  SoaPos pos;      // lambda (one lane per particle) reads its candidates here;
+                   // delta writes its new predictions here when not null
 };
 constexpr int kMaskMaxRow = 64;
 
@@ -1123,6 +1151,78 @@ __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __res
 #pragma unroll
     for (int u = 1; u < B; ++u)
       if (u < rem && f(k + u, d[u])) bits |= 1u << u;
+    acc |= (uint64_t)bits << (k - S[0]);
+    k += B;
+    next_row();
+  }
+  rm[(8 - left) * stride] = acc;
+  return true;
+}
+
+// visit_record reading its candidates from the x / y / z arrays, four slots
+// to a 16-byte load (the position's w is not needed by lambda): B candidates
+// in 3B/4 load instructions instead of B.  On the 1.64M scene, lambda 331 ->
+// 302 us with 8 candidates per batch (72 registers, 7 waves per SIMD); 4: 334
+// us, 12: 332, 16: 341.  The candidates reach f in the same order as in
+// visit_record.
+template <int B, class F>
+__device__ __forceinline__ bool visit_record_xyz(const Grid& g, const int32_t* __restrict__ off, int max_scan,
+                                                 int bx, int by, int bz, const SoaPos& a, F&& f, uint64_t* rm,
+                                                 int stride) {
+  static_assert(B % 4 == 0 && B <= 16, "whole 16-byte loads within the arrays' padding");
+  struct __attribute__((aligned(4))) F4 {
+    float v[4];
+  };
+  const int x0 = max(bx - 1, 0), x1 = min(bx + 1, g.cx - 1);
+  int S[9];
+  uint32_t Lp[5];
+  bool big = false;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
+    const bool in = z >= 0 && z < g.cz && y >= 0 && y < g.cy;
+    const int c0 = in ? (z * g.cy + y) * g.cx + x0 : 0;
+    S[r] = in ? off[c0] : 0;
+    const int len = in ? off[c0 + (x1 - x0) + 1] - S[r] : 0;
+    big |= len > max_scan || len > kMaskMaxRow;
+    if (r % 2 == 0)
+      Lp[r / 2] = (uint32_t)len;
+    else
+      Lp[r / 2] |= (uint32_t)len << 16;
+  }
+  if (big) return false;
+  int k = S[0], e = S[0] + (int)(Lp[0] & 0xFFFFu), left = 8;
+  uint64_t acc = 0;
+  auto next_row = [&]() {
+    while (k >= e && left > 0) {
+      rm[(8 - left) * stride] = acc;
+      acc = 0;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) S[r] = S[r + 1];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Lp[i] = (Lp[i] >> 16) | (Lp[i + 1] << 16);
+      Lp[4] >>= 16;
+      k = S[0];
+      e = S[0] + (int)(Lp[0] & 0xFFFFu);
+      --left;
+    }
+  };
+  next_row();
+#pragma unroll 1
+  while (k < e) {
+    F4 X[B / 4], Y[B / 4], Z[B / 4];
+#pragma unroll
+    for (int i = 0; i < B / 4; ++i) {
+      X[i] = *reinterpret_cast<const F4*>(a.x + k + 4 * i);
+      Y[i] = *reinterpret_cast<const F4*>(a.y + k + 4 * i);
+      Z[i] = *reinterpret_cast<const F4*>(a.z + k + 4 * i);
+    }
+    const int rem = e - k;
+    uint32_t bits = f(k, make_float4(X[0].v[0], Y[0].v[0], Z[0].v[0], 0.0f)) ? 1u : 0u;
+#pragma unroll
+    for (int u = 1; u < B; ++u)
+      if (u < rem && f(k + u, make_float4(X[u / 4].v[u % 4], Y[u / 4].v[u % 4], Z[u / 4].v[u % 4], 0.0f)))
+        bits |= 1u << u;
     acc |= (uint64_t)bits << (k - S[0]);
     k += B;
     next_row();
@@ -1262,6 +1362,9 @@ constexpr int kLdsWalk = 1024;
 // iteration walks only those.
 constexpr int kMaskRecord = 2048;
 constexpr int kMaskWalk = 2048;
+// Kernel template bit (with kMaskRecord): lambda reads its candidates from
+// the x / y / z arrays (PairMasks::pos, visit_record_xyz).
+constexpr int kXyzCand = 4096;
 constexpr int kTileBlock = 256;
 constexpr int kTileCap = 2496;  // float4 slots: four workgroups share a CU's 160 KB
 constexpr int kTileGlobal = std::numeric_limits<int>::min();
@@ -1875,8 +1978,11 @@ __device__ __forceinline__ void lambda_body(int gt_in, int n, Grid g_, Consts k_
         };
         // (4 loads per batch in the batched pass: 73 -> 69 registers, 7 waves per SIMD;
         // 6 in the plain pass: 331 -> 334 us)
-        const bool recorded =
-            visit_record<(V & 8) ? 4 : 5>(g, off, k.max_scan, bx, by, bz, ld, body_rec, rm, (int)blockDim.x);
+        bool recorded;
+        if constexpr ((V & kXyzCand) != 0)
+          recorded = visit_record_xyz<8>(g, off, k.max_scan, bx, by, bz, pm.pos, body_rec, rm, (int)blockDim.x);
+        else
+          recorded = visit_record<(V & 8) ? 4 : 5>(g, off, k.max_scan, bx, by, bz, ld, body_rec, rm, (int)blockDim.x);
         if (recorded) {
 #pragma unroll
           for (int r = 0; r < 9; ++r) pm.rows[(size_t)r * n + t] = rm[r * (int)blockDim.x];
@@ -2041,6 +2147,7 @@ __device__ __forceinline__ void delta_body(int gt_in, int n, Grid g_, Consts k_,
     const float2 dl = dl_s[t];
     const float4 pn = make_float4(pi.x + ddx, pi.y + ddy, pi.z + ddz, dl.x);
     if (pred_out) pred_out[t] = pn;
+    if constexpr ((V & kMaskWalk) != 0) soa_store(pm.pos, t, pn.x, pn.y, pn.z);
     if (vel_s) {
       const float dtc = fmaxf(k.dt, 1e-6f);
       const float4 po = pos_s[t];
@@ -2641,7 +2748,14 @@ static Lists make_lists(const Ws& w, const Consts& k, bool enabled) {
 // asks for the walks without them (walk 4).  Their words take the lists' mask
 // area and whole-stencil flags.
 static bool pair_masks_on(bool lists) { return !lists && g_tuning.walk == 0; }
-static PairMasks pair_masks(const Ws& w) { return PairMasks{w.mask, w.lfull}; }
+static PairMasks pair_masks(const Ws& w, bool xyz = false) {
+  return PairMasks{w.mask, w.lfull, xyz ? SoaPos{w.sx, w.sy, w.sz} : SoaPos{nullptr, nullptr, nullptr}};
+}
+// lambda's one-lane recording walk reads the x / y / z arrays (visit_record_xyz)
+static bool xyz_candidates(long long n, bool lists) { return pair_masks_on(lists) && n >= kQuadBelowLambda; }
+static SoaPos xyz_arrays(const Ws& w, long long n, bool lists) {
+  return xyz_candidates(n, lists) ? SoaPos{w.sx, w.sy, w.sz} : SoaPos{nullptr, nullptr, nullptr};
+}
 // nine words per lane, or per quad of lanes in the quad walk
 static size_t row_words_lds(int block, bool quad = false) {
   return (size_t)9 * sizeof(uint64_t) * (size_t)(quad ? block / 4 : block);
@@ -2710,7 +2824,7 @@ static int build_grid(const FxParams& p, FxState& s, const Ws& w, bool do_predic
   if (n > 0) {
     LAUNCH(k_gather, n, st, n, w.sidx, key, (const float4*)s.pred, (const float4*)s.pos,
            (const float4*)nullptr, w.pred_s[target], w.pos_s, w.dl_s, s.sorted_key, w.skey,
-           s.sorted_idx);
+           s.sorted_idx, xyz_arrays(w, n, g_tuning.skin > 0.0f));
     if (s.key) FX_HIP_CHECK(hipMemcpyAsync(s.key, key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
   }
   // a new grid invalidates the neighbour lists until a lambda pass rebuilds
@@ -2741,7 +2855,8 @@ static int build_grid_resident(const FxParams& p, FxState& s, const Ws& w, int t
   FX_TRY(canon_pairs(w.off, w.pair64, nc, w.rank, w.flags, st));
   LAUNCH(k_gather_pairs, n, st, n, (const uint64_t*)w.pair64, (const int32_t*)w.key, pred_r,
          (const float4*)w.pos_r, w.pred_s[target], w.pos_s, w.dl_s, last_in_call ? s.sorted_key : nullptr, w.skey,
-         w.sidx, last_in_call ? s.sorted_idx : nullptr, last_in_call ? s.key : nullptr);
+         w.sidx, last_in_call ? s.sorted_idx : nullptr, last_in_call ? s.key : nullptr,
+         xyz_arrays(w, n, g_tuning.skin > 0.0f));
   return check_launch("build_grid_resident");
 }
 
@@ -2802,7 +2917,8 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
     FX_TRY(fill(w.flags + 2, 1, 0, st));
   }
   const bool masks = pair_masks_on(lists);
-  const PairMasks pm = pair_masks(w);
+  const bool xyz = xyz_candidates(n, lists);
+  const PairMasks pm = pair_masks(w, xyz);
   for (int it = 0; it < iters; ++it) {
     const bool last = it == iters - 1;
     float4* lam_out = (last && last_in_call && s.lambda) ? (float4*)s.lambda : (float4*)nullptr;
@@ -2811,9 +2927,9 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
                     dim3(g_tuning.block), row_words_lds(g_tuning.block, true), st, n, g, k, w.pred_s[cur], w.off,
                     w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0}, -1, (const int32_t*)nullptr, pm);
     else if (masks)
-      FX_LAUNCH_SHM(k_lambda<VISIT_FULL + kMaskRecord>, dim3(grid_for(n, g_tuning.block)), dim3(g_tuning.block),
-                    row_words_lds(g_tuning.block), st, n, g, k, w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L,
-                    Scenes{nullptr, 0}, -1, (const int32_t*)nullptr, pm);
+      FX_LAUNCH_SHM(k_lambda<VISIT_FULL + kMaskRecord + kXyzCand>, dim3(grid_for(n, g_tuning.block)),
+                    dim3(g_tuning.block), row_words_lds(g_tuning.block), st, n, g, k, w.pred_s[cur], w.off,
+                    w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0}, -1, (const int32_t*)nullptr, pm);
     else if (!lists && g_tuning.walk == 3 && n >= kRowBelow)
       FX_LAUNCH(k_lambda<VISIT_FULL + kLdsWalk>, dim3(grid_for(n, kTileBlock)), dim3(kTileBlock), st, n, g, k,
                 w.pred_s[cur], w.off, w.dl_s, w.sidx, lam_out, L, Scenes{nullptr, 0}, -1, (const int32_t*)w.skey);
@@ -2828,10 +2944,11 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
                 dim3(g_tuning.block), st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
                 w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
                 (float4*)nullptr, L, (const int32_t*)nullptr, Scenes{nullptr, 0}, -1, (const int32_t*)nullptr, pm);
-    else if (masks)
+    else if (masks)  // the next lambda's x / y / z candidates, unless this is the last iteration
       NLAUNCH(k_delta<VISIT_FULL + kMaskWalk>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s,
               w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
-              (float4*)nullptr, L, (const int32_t*)nullptr, Scenes{nullptr, 0}, -1, (const int32_t*)nullptr, pm);
+              (float4*)nullptr, L, (const int32_t*)nullptr, Scenes{nullptr, 0}, -1, (const int32_t*)nullptr,
+              pair_masks(w, xyz && !last));
     else if (lists)
       NLAUNCH(k_delta<VISIT_LIST>, n, st, n, g, k, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off,
               w.pred_s[cur ^ 1], (const float4*)w.pos_s, last ? w.vel_s : (float4*)nullptr, w.sidx,
