@@ -1164,7 +1164,9 @@ __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __res
 // in 3B/4 load instructions instead of B.  On the 1.64M scene, lambda 331 ->
 // 302 us with 8 candidates per batch (72 registers, 7 waves per SIMD); 4: 334
 // us, 12: 332, 16: 341.  The candidates reach f in the same order as in
-// visit_record.
+// visit_record.  In the batched pass (per-scene constants in registers) it
+// holds 84 registers, 6 waves per SIMD, and measured slower (1186 -> 1236 us
+// on the 64-scene batch), so the batch keeps visit_record.
 template <int B, class F>
 __device__ __forceinline__ bool visit_record_xyz(const Grid& g, const int32_t* __restrict__ off, int max_scan,
                                                  int bx, int by, int bz, const SoaPos& a, F&& f, uint64_t* rm,
