@@ -1166,7 +1166,10 @@ __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __res
 // us, 12: 332, 16: 341.  The candidates reach f in the same order as in
 // visit_record.  In the batched pass (per-scene constants in registers) it
 // holds 84 registers, 6 waves per SIMD, and measured slower (1186 -> 1236 us
-// on the 64-scene batch), so the batch keeps visit_record.
+// on the 64-scene batch), so the batch keeps visit_record.  Loading each
+// row's bounds only while the row before is walked (after an up-front check
+// of the nine lengths) holds 62 registers instead of 72 but took 301 -> 320
+// us: the row changes then wait on those loads.
 template <int B, class F>
 __device__ __forceinline__ bool visit_record_xyz(const Grid& g, const int32_t* __restrict__ off, int max_scan,
                                                  int bx, int by, int bz, const SoaPos& a, F&& f, uint64_t* rm,
