@@ -1169,7 +1169,10 @@ __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __res
 // on the 64-scene batch), so the batch keeps visit_record.  Loading each
 // row's bounds only while the row before is walked (after an up-front check
 // of the nine lengths) holds 62 registers instead of 72 but took 301 -> 320
-// us: the row changes then wait on those loads.
+// us: the row changes then wait on those loads.  Recording alone (the
+// distance test, no terms) and then walking the recorded words for the sums
+// as delta does is bit-identical but holds 82 registers (5 waves per SIMD)
+// and took 560 us: the scan, not the in-range terms, sets the pass's time.
 template <int B, class F>
 __device__ __forceinline__ bool visit_record_xyz(const Grid& g, const int32_t* __restrict__ off, int max_scan,
                                                  int bx, int by, int bz, const SoaPos& a, F&& f, uint64_t* rm,
@@ -2264,7 +2267,13 @@ __device__ __forceinline__ void xsph_body(int gt_in, int n, FxParams p_, Grid g_
       if (gt & 3) return;
     } else {
       // stage 1 loads positions; stage 2 loads velocities only for candidates
-      // within h (predicated, still before any use); stage 3 accumulates
+      // within h (predicated, still before any use); stage 3 accumulates.
+      // Positions from the x / y / z arrays instead (written by the last
+      // delta; bit-identical), on the 1.64M scene: row batches of 4 slots (3
+      // loads) 560 us, of 8 718, three scalar loads per candidate in this
+      // stream 1013, against 520 here (row batches save position loads but
+      // add partly empty batches at row ends, each still issuing its
+      // velocity loads).
       auto ld = [&](int q) {
         PV c;
         c.p = pred_s[q];
