@@ -1173,6 +1173,8 @@ __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __res
 // distance test, no terms) and then walking the recorded words for the sums
 // as delta does is bit-identical but holds 82 registers (5 waves per SIMD)
 // and took 560 us: the scan, not the in-range terms, sets the pass's time.
+// Held to 64 registers for 8 waves per SIMD (amdgpu_waves_per_eu(8)), four
+// values spill to scratch: 320 us.
 template <int B, class F>
 __device__ __forceinline__ bool visit_record_xyz(const Grid& g, const int32_t* __restrict__ off, int max_scan,
                                                  int bx, int by, int bz, const SoaPos& a, F&& f, uint64_t* rm,
@@ -2273,7 +2275,8 @@ __device__ __forceinline__ void xsph_body(int gt_in, int n, FxParams p_, Grid g_
       // loads) 560 us, of 8 718, three scalar loads per candidate in this
       // stream 1013, against 520 here (row batches save position loads but
       // add partly empty batches at row ends, each still issuing its
-      // velocity loads).
+      // velocity loads).  Loading every candidate's velocity with its
+      // position, without waiting for the range test: 749 us.
       auto ld = [&](int q) {
         PV c;
         c.p = pred_s[q];
