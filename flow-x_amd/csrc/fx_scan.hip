@@ -1,5 +1,6 @@
 // Device-wide exclusive scan of int32 counts (reduce -> scan partials ->
-// scan tiles).  Used for cell offsets (counting sort) and surface compaction.
+// scan tiles; up to kFoldTiles tiles the last step scans the partials too).
+// Used for cell offsets (counting sort) and surface compaction.
 #include "fx_common.h"
 
 namespace fx {
@@ -103,6 +104,49 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const int32_t* __re
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partial[nb];
 }
 
+// k_scan_final that scans the tile sums itself: tile b adds up partial[0..b)
+// (the last tile all of them, for the total), so no launch scans the
+// partials in between.  Up to kFoldTiles tiles (each block then reads at
+// most kFoldTiles / 256 sums per thread).
+constexpr int kFoldTiles = 2048;
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_final_fold(const int32_t* __restrict__ in,
+                                                                   int32_t* __restrict__ out, long long n,
+                                                                   const int32_t* __restrict__ partial,
+                                                                   int nb) {
+  __shared__ int lds[4];
+  const int b = (int)blockIdx.x;
+  const bool last = b == nb - 1;
+  int before = 0, all = 0;
+  for (int i = (int)threadIdx.x; i < (last ? nb : b); i += kScanThreads) {
+    const int v = partial[i];
+    if (i < b) before += v;
+    all += v;
+  }
+  int prefix, total;
+  block_inclusive_scan(before, lds, &prefix);
+  block_inclusive_scan(all, lds, &total);
+  long long base = (long long)b * kScanTile + (long long)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    long long q = base + i;
+    v[i] = q < n ? in[q] : 0;
+    s += v[i];
+  }
+  int tot;
+  int incl = block_inclusive_scan(s, lds, &tot);
+  int run = prefix + incl - s;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    long long q = base + i;
+    if (q < n) out[q] = run;
+    run += v[i];
+  }
+  if (last && threadIdx.x == 0) out[n] = total;
+}
+
 // Up to kSmallScanMax items, one block scans everything in one launch: a
 // small scene's cell offsets then cost one launch instead of three.  The
 // counts pass through LDS so that global loads and stores stay coalesced
@@ -177,6 +221,11 @@ int exclusive_scan_i32(const int32_t* counts, int32_t* out, long long n, int32_t
   if (nb < 1) nb = 1;
   if (nb > kMaxPartials) return fail(FX_ERANGE, "scan of %lld items exceeds %d tiles", n, kMaxPartials);
   FX_LAUNCH(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), st, counts, n, partial);
+  if (nb <= kFoldTiles) {
+    FX_LAUNCH(k_scan_final_fold, dim3((unsigned)nb), dim3(kScanThreads), st, counts, out, n,
+              (const int32_t*)partial, (int)nb);
+    return check_launch("exclusive_scan_i32");
+  }
   FX_LAUNCH(k_scan_partials, dim3(1), dim3(1024), st, partial, (int)nb);
   FX_LAUNCH(k_scan_final, dim3((unsigned)nb), dim3(kScanThreads), st, counts, out, n, (const int32_t*)partial,
             (int)nb);

@@ -534,6 +534,9 @@ __device__ __forceinline__ void soa_store(const SoaPos& a, int t, float x, float
 // The gather of a resident build: slot u of the new order reads its previous
 // slot (mostly a nearby one: particles rarely change cells), and records the
 // particle in sidx (plus the caller's reference-format copies when given).
+// dl_s (zeroed: read before any lambda pass only with no iterations) and the
+// sorted keys skey_int (read by the lists and the LDS-tiled walk) are
+// skipped when null.
 __global__ void k_gather_pairs(int n, const uint64_t* __restrict__ pairs, const int32_t* __restrict__ key_r,
                                const float4* __restrict__ pred_r, const float4* __restrict__ pos_r,
                                float4* __restrict__ pred_s, float4* __restrict__ pos_s,
@@ -551,13 +554,15 @@ __global__ void k_gather_pairs(int n, const uint64_t* __restrict__ pairs, const 
   pred_s[u] = q;
   soa_store(soa, u, q.x, q.y, q.z);
   pos_s[u] = pos_r[src];
-  dl_s[u] = make_float2(0.0f, 0.0f);
-  const int k = key_r[src];
-  skey_int[u] = k;
-  if (skey_out) skey_out[u] = k;
+  if (dl_s) dl_s[u] = make_float2(0.0f, 0.0f);
   sidx[u] = pid;
   if (sidx_out) sidx_out[u] = pid;
-  if (key_canon) key_canon[pid] = k;
+  if (skey_int || skey_out || key_canon) {
+    const int k = key_r[src];
+    if (skey_int) skey_int[u] = k;
+    if (skey_out) skey_out[u] = k;
+    if (key_canon) key_canon[pid] = k;
+  }
 }
 
 // Canonicalise each cell's run to ascending particle index.  Runs of up to
@@ -2870,10 +2875,11 @@ static int build_grid_resident(const FxParams& p, FxState& s, const Ws& w, int t
          (const int32_t*)w.off, (const int32_t*)w.sidx, w.pair64, w.cnt, nc + 1, w.flags);
   // big-cell queue in the rank array (dead after the scatter)
   FX_TRY(canon_pairs(w.off, w.pair64, nc, w.rank, w.flags, st));
+  const bool lists = g_tuning.skin > 0.0f;
   LAUNCH(k_gather_pairs, n, st, n, (const uint64_t*)w.pair64, (const int32_t*)w.key, pred_r,
-         (const float4*)w.pos_r, w.pred_s[target], w.pos_s, w.dl_s, last_in_call ? s.sorted_key : nullptr, w.skey,
-         w.sidx, last_in_call ? s.sorted_idx : nullptr, last_in_call ? s.key : nullptr,
-         xyz_arrays(w, n, g_tuning.skin > 0.0f));
+         (const float4*)w.pos_r, w.pred_s[target], w.pos_s, p.iterations > 0 ? (float2*)nullptr : w.dl_s,
+         last_in_call ? s.sorted_key : nullptr, (lists || g_tuning.walk == 3) ? w.skey : (int32_t*)nullptr,
+         w.sidx, last_in_call ? s.sorted_idx : nullptr, last_in_call ? s.key : nullptr, xyz_arrays(w, n, lists));
   return check_launch("build_grid_resident");
 }
 

@@ -89,11 +89,14 @@ def test_build_grid_matches_oracle(fx, oracle, scene):
     assert np.array_equal(g.get("cell_end"), end)
 
 
-@pytest.mark.parametrize("dims", [(8, 8, 16), (5, 5, 41), (16, 32, 32), (5, 29, 113)])
+@pytest.mark.parametrize("dims", [(8, 8, 16), (5, 5, 41), (16, 32, 32), (5, 29, 113), (32, 64, 2048),
+                                  (32, 64, 2049)])
 def test_build_grid_at_scan_size_boundaries(fx, oracle, scene, dims):
     """Cell counts of 1024 / 1025 and 16384 / 16385 straddle the one-block
     scan's direct form (one count per thread), its LDS-staged form and the
-    multi-block scan.  The dam-break state binned into each grid
+    multi-block scan; 4,194,304 / 4,196,352 cells (2048 / 2049 tiles of
+    2048) straddle the multi-block scan whose last launch sums the tile
+    totals itself and the one with a launch between.  The dam-break state binned into each grid
     (coordinates clamp to the dims, so the small grids hold runs of
     thousands for the block-wide sort) must give the oracle's order and
     ranges."""
