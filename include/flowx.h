@@ -50,7 +50,8 @@ extern "C" {
  * lane of the reference's 128-byte push-constant block
  * (shaders/sph_common.glsl:10-17, solver/sph.py:433-461).  The two values the
  * reference bit-packs into i_sort.zw (scorr_k, surface_tension) and
- * i_collider.w (voxel size) are plain floats here. */
+ * i_collider.w (voxel size) are plain floats here.  n is at most 2^28 - 1
+ * (a batch's total too): larger counts are refused with FX_ERANGE. */
 typedef struct FxParams {
   int32_t n;                 /* i_layout.x  particle count                        */
   int32_t cells[3];          /* i_grid.xyz  cell dims (x fastest)                 */

@@ -59,6 +59,11 @@ def test_invalid_arguments_return_codes_without_gpu(fx):
     assert rc == -1 and b"cell dims" in L.lib().fx_last_error()
     with pytest.raises(RuntimeError):
         L.check(rc, "fx_build_grid")
+    # slots are addressed by 32-bit byte offsets: 2^28 particles are refused
+    p = cfg.params(0.01)
+    p.n = 1 << 28
+    rc = L.lib().fx_build_grid(ctypes.byref(p), ctypes.byref(s), None)
+    assert rc == -4 and b"particle count" in L.lib().fx_last_error()
 
 
 def test_mc_table_of_library_matches_reference_golden(fx):
