@@ -2304,7 +2304,10 @@ __device__ __forceinline__ void xsph_body(int gt_in, int n, FxParams p_, Grid g_
       // stream 1013, against 520 here (row batches save position loads but
       // add partly empty batches at row ends, each still issuing its
       // velocity loads).  Loading every candidate's velocity with its
-      // position, without waiting for the range test: 749 us.
+      // position, without waiting for the range test: 749 us.  Each lane's
+      // nine row runs kept in LDS instead of the register queue (a row change
+      // one LDS read instead of a shift unrolled at every slot): 69 instead
+      // of 72 registers, 517 against 522 us; not kept.
       auto ld = [&](int q) {
         PV c;
         c.p = pred_s[q];
