@@ -2377,6 +2377,13 @@ __device__ __forceinline__ void xsph_body(int gt_in, int n, FxParams p_, Grid g_
   if (pos_s) pos_s[t] = pf;
 }
 
+// XSPH's blocks in one contiguous range per XCD (xcd_block) against
+// round-robin: 515 / 521 us at 1.64M particles, but the 4.1M animated scene's
+// substep takes 10.30 / 9.95 ms (lambda and delta measured slower with it at
+// both sizes: 1.64M lambda 302 -> 319 us, delta 131 -> 144; 4.1M substep
+// 10.3 -> 18.4 ms)
+constexpr int kXsphXcdBelow = 1 << 21;
+
 template <int V>
 __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Consts k_,
                                               Collider col_, const float4* __restrict__ pred_s,
@@ -2394,7 +2401,9 @@ __global__ __launch_bounds__(256) void k_xsph(int n, FxParams p_, Grid g_, Const
     for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
       xsph_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, p_, g_, k_, col_, pred_s, vel_s, off_, sidx, delta4, pos4, vel4, pos_s, pred4_out, L, S, normal4, normal_by_slot);
   } else {
-    xsph_body<V>(xcd_block() * blockDim.x + threadIdx.x, n, p_, g_, k_, col_, pred_s, vel_s, off_, sidx, delta4, pos4, vel4, pos_s, pred4_out, L, S, normal4, normal_by_slot);
+    // blocks renumbered per XCD below kXsphXcdBelow particles only
+    const int b = n < kXsphXcdBelow ? xcd_block() : (int)blockIdx.x;
+    xsph_body<V>(b * blockDim.x + threadIdx.x, n, p_, g_, k_, col_, pred_s, vel_s, off_, sidx, delta4, pos4, vel4, pos_s, pred4_out, L, S, normal4, normal_by_slot);
   }
 }
 
