@@ -55,6 +55,12 @@ def algorithmic_bytes(kernel, n, c):
     return per_particle * n, per_particle
 
 
+# register count and the occupancy it allows (512 VGPRs per lane of a SIMD)
+# of the kernels the roofline can name, from the gfx950 code objects
+WAVES_PER_SIMD = {"k_lambda<VISIT_FULL + kMaskRecord + kXyzCand>": (72, 7), "k_lambda<VISIT_FULL>": (54, 8),
+                  "k_delta<VISIT_FULL + kMaskWalk>": (61, 8), "k_xsph<32 + 16 + VISIT_FULL>": (72, 7)}
+
+
 def profile_name(kernel):
     """File-name form of a kernel's launch label: 'k_lambda<VISIT_FULL + kMaskRecord>' ->
     'k_lambda_VISIT_FULL_kMaskRecord'."""
@@ -754,8 +760,10 @@ def main():
             # limits the kernel is each wave's instruction stream and its
             # dependent loads at the occupancy ceiling, with the
             # vector-memory pipeline near saturation
+            occ = WAVES_PER_SIMD.get(dominant)
+            at = f" at {occ[1]} waves per SIMD ({occ[0]} VGPRs)" if occ else ""
             roofline["measured_limiter"] = (
-                f"per-wave instruction issue and load latency at 8 waves per SIMD: VALU "
+                f"per-wave instruction issue and load latency{at}: VALU "
                 f"{ctr.get('valu_insts_per_cu_cycle', 0):.2f} instructions per CU per cycle, "
                 f"{ctr.get('wait_inst_share', 0) * 100:.0f}% of wave cycles waiting on a dependency, TA "
                 f"{ctr.get('ta_busy', 0) * 100:.0f}% / TD {ctr.get('td_busy', 0) * 100:.0f}% busy, VALU lanes "
