@@ -547,8 +547,8 @@ __device__ __forceinline__ void soa_store(const SoaPos& a, int t, float x, float
 // slot (mostly a nearby one: particles rarely change cells), and records the
 // particle in sidx (plus the caller's reference-format copies when given).
 // dl_s (zeroed: read before any lambda pass only with no iterations) and the
-// sorted keys skey_int (read by the lists and the LDS-tiled walk) are
-// skipped when null.
+// sorted keys skey_int (read by the lists, the LDS-tiled walk and, after the
+// call, the splat) are skipped when null.
 __global__ void k_gather_pairs(int n, const uint64_t* __restrict__ pairs, const int32_t* __restrict__ key_r,
                                const float4* __restrict__ pred_r, const float4* __restrict__ pos_r,
                                float4* __restrict__ pred_s, float4* __restrict__ pos_s,
@@ -2913,11 +2913,15 @@ static int build_grid_resident(const FxParams& p, FxState& s, const Ws& w, int t
          (const int32_t*)w.off, (const int32_t*)w.sidx, w.pair64, w.cnt, nc + 1, w.flags);
   // big-cell queue in the rank array (dead after the scatter)
   FX_TRY(canon_pairs(w.off, w.pair64, nc, w.rank, w.flags, st));
+  // the sorted keys: for the lists and the LDS-tiled walk, and after the
+  // call's last substep for the splat (its per-cell displacement bounds bin
+  // each slot by its key)
   const bool lists = g_tuning.skin > 0.0f;
+  const bool skey = lists || g_tuning.walk == 3 || last_in_call;
   LAUNCH(k_gather_pairs, n, st, n, (const uint64_t*)w.pair64, (const int32_t*)w.key, pred_r,
          (const float4*)w.pos_r, w.pred_s[target], w.pos_s, p.iterations > 0 ? (float2*)nullptr : w.dl_s,
-         last_in_call ? s.sorted_key : nullptr, (lists || g_tuning.walk == 3) ? w.skey : (int32_t*)nullptr,
-         w.sidx, last_in_call ? s.sorted_idx : nullptr, last_in_call ? s.key : nullptr, xyz_arrays(w, n, lists));
+         last_in_call ? s.sorted_key : nullptr, skey ? w.skey : (int32_t*)nullptr, w.sidx,
+         last_in_call ? s.sorted_idx : nullptr, last_in_call ? s.key : nullptr, xyz_arrays(w, n, lists));
   return check_launch("build_grid_resident");
 }
 
