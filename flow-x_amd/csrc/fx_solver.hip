@@ -1191,7 +1191,9 @@ __device__ __forceinline__ bool visit_record(const Grid& g, const int32_t* __res
 // as delta does is bit-identical but holds 82 registers (5 waves per SIMD)
 // and took 560 us: the scan, not the in-range terms, sets the pass's time.
 // Held to 64 registers for 8 waves per SIMD (amdgpu_waves_per_eu(8)), four
-// values spill to scratch: 320 us.
+// values spill to scratch: 320 us.  Branch-free forms of the per-slot tests
+// (slots past the run given x = +inf, the own-particle gradient test as a
+// select) took 314 / 313 us, both 320: the branches let whole waves skip.
 template <int B, class F>
 __device__ __forceinline__ bool visit_record_xyz(const Grid& g, const int32_t* __restrict__ off, int max_scan,
                                                  int bx, int by, int bz, const SoaPos& a, F&& f, uint64_t* rm,
