@@ -58,7 +58,7 @@ def algorithmic_bytes(kernel, n, c):
 # register count and the occupancy it allows (512 VGPRs per lane of a SIMD)
 # of the kernels the roofline can name, from the gfx950 code objects
 WAVES_PER_SIMD = {"k_lambda<VISIT_FULL + kMaskRecord + kXyzCand>": (72, 7), "k_lambda<VISIT_FULL>": (54, 8),
-                  "k_delta<VISIT_FULL + kMaskWalk>": (61, 8), "k_xsph<32 + 16 + VISIT_FULL>": (72, 7)}
+                  "k_delta<VISIT_FULL + kMaskWalk + kLdsWords>": (61, 8), "k_xsph<32 + 16 + VISIT_FULL>": (72, 7)}
 
 
 def profile_name(kernel):
