@@ -2273,15 +2273,17 @@ __global__ __launch_bounds__(256) void k_delta(int n, Grid g_, Consts k_,
                                                const int32_t* __restrict__ skey, Scenes S,
                                                int iter = -1, const int32_t* __restrict__ tkey = nullptr,
                                                PairMasks pm = PairMasks{nullptr, nullptr}) {
+  // kLdsWords: each lane's nine row words, row r at
+  // fx_delta_words[r * blockDim.x + lane]
+  extern __shared__ uint64_t fx_delta_words[];
+  uint64_t* const rt = ((V & kLdsWords) != 0) ? fx_delta_words + threadIdx.x : nullptr;
   if constexpr ((V & kPersistent) != 0) {
     constexpr bool QUADW = (V & kQuadWalk) != 0;
     const long long nth = (QUADW ? 4LL * n : (long long)n);
     for (int vb = persistent_take(S.queue, nth); vb >= 0; vb = persistent_take(S.queue, nth))
-      delta_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter, nullptr, pm);
+      delta_body<V>(vb * (int)blockDim.x + (int)threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter, nullptr, pm, rt);
   } else if constexpr ((V & kLdsWords) != 0) {
-    // each lane's nine row words, row r at fx_delta_words[r * blockDim.x + lane]
-    extern __shared__ uint64_t fx_delta_words[];
-    delta_body<V>(blockIdx.x * blockDim.x + threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter, nullptr, pm, fx_delta_words + threadIdx.x);
+    delta_body<V>(blockIdx.x * blockDim.x + threadIdx.x, n, g_, k_, pred_in, dl_s, off_, pred_out, pos_s, vel_s, sidx, delta4, L, skey, S, iter, nullptr, pm, rt);
   } else if constexpr ((V & kLdsWalk) != 0) {
     // launched with kTileBlock threads; tkey: the sorted build keys
     __shared__ Tile tile;
@@ -3410,9 +3412,10 @@ static int batch_step(const FxScene* scenes, int count, float* pos, float* vel, 
     if (masks) {
       BLAUNCH_SHM(k_lambda, 8 + VISIT_FULL + kMaskRecord, n, row_words_lds(g_tuning.block), st, n, Grid{}, Consts{},
                   w.pred_s[cur], w.off, w.dl_s, w.sidx, (float4*)nullptr, L0, S, it, (const int32_t*)nullptr, pm);
-      BLAUNCH(k_delta, 8 + VISIT_FULL + kMaskWalk, n, st, n, Grid{}, Consts{}, (const float4*)w.pred_s[cur],
-              (const float2*)w.dl_s, w.off, w.pred_s[cur ^ 1], (const float4*)w.pos_s, w.vel_s, w.sidx,
-              (float4*)nullptr, L0, (const int32_t*)nullptr, S, it, (const int32_t*)nullptr, pm);
+      BLAUNCH_SHM(k_delta, 8 + VISIT_FULL + kMaskWalk + kLdsWords, n, row_words_lds(g_tuning.block), st, n, Grid{},
+                  Consts{}, (const float4*)w.pred_s[cur], (const float2*)w.dl_s, w.off, w.pred_s[cur ^ 1],
+                  (const float4*)w.pos_s, w.vel_s, w.sidx, (float4*)nullptr, L0, (const int32_t*)nullptr, S, it,
+                  (const int32_t*)nullptr, pm);
     } else {
       BLAUNCH(k_lambda, 8 + VISIT_FULL, n, st, n, Grid{}, Consts{}, w.pred_s[cur], w.off, w.dl_s, w.sidx,
               (float4*)nullptr, L0, S, it);
