@@ -1283,70 +1283,23 @@ __device__ __forceinline__ void visit_cells_capped(const Grid& g, const int32_t*
   }
 }
 
+
 // Delta's walk over the recorded words: B set bits of the current row per
 // batch (the batch's loads issued together, empty slots re-loading its
 // first), the candidates reaching f in slot order, each lane running on to
-// its next row without waiting for its wave at row ends (the nine words and
-// first slots shift down a queue as the lane leaves a row, which keeps them
-// in registers).  On the 1.64M scene (delta us): 134 with 4 bits per batch,
-// 141 / 137 with 3 / 6; walked row by row, the wave waiting at each row's
-// end for its lane with the most bits, 152 (3 / 6 / 8 bits: 156 / 159 /
-// 165), and with only the next row's word loaded ahead 155; a z-plane of
-// three rows taken as one run, a batch's bits picked across the plane's rows
-// by selects, 155-166.  The 64-scene batch: 565 against 592 row by row.
-template <int B, class LD, class F>
-__device__ __forceinline__ void visit_masked(const Grid& g, const int32_t* __restrict__ off, int bx, int by,
-                                             int bz, int t, int n, const uint64_t* __restrict__ rows, LD&& ld,
-                                             F&& f) {
-  const int x0 = max(bx - 1, 0);
-  int S[9];
-  uint64_t M[9];
-#pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const int z = bz + r / 3 - 1, y = by + r % 3 - 1;
-    const bool in = z >= 0 && z < g.cz && y >= 0 && y < g.cy;
-    S[r] = in ? off[(z * g.cy + y) * g.cx + x0] : 0;
-    M[r] = rows[(size_t)r * n + t];
-  }
-  int left = 8;
-  auto next_row = [&]() {
-    while (M[0] == 0 && left > 0) {
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        M[r] = M[r + 1];
-        S[r] = S[r + 1];
-      }
-      M[8] = 0;
-      --left;
-    }
-  };
-  next_row();
-#pragma unroll 1
-  while (M[0]) {
-    int q[B];
-    bool v[B];
-#pragma unroll
-    for (int u = 0; u < B; ++u) {
-      v[u] = M[0] != 0;
-      q[u] = S[0] + __ffsll((unsigned long long)M[0]) - 1;  // (S[0] - 1 once the row is empty: not loaded)
-      M[0] &= M[0] - 1;
-    }
-    decltype(ld(0)) d[B];
-#pragma unroll
-    for (int u = 0; u < B; ++u) d[u] = ld(v[u] ? q[u] : q[0]);
-#pragma unroll
-    for (int u = 0; u < B; ++u)
-      if (v[u]) f(q[u], d[u]);
-    next_row();
-  }
-}
-
-// visit_masked with the nine row words parked in LDS (rt: the lane's first,
-// row r at rt[r * lstride]): a row change reads the next word instead of
-// shifting eighteen registers of words down the queue (the row starts still
-// shift).  The same candidates in the same order; on the 1.64M scene delta
-// 131 -> 120 us (VALU instructions per wave 3907 -> 3267, share of wave
-// cycles waiting on a dependency 41% -> 26%).
+// its next row without waiting for its wave at row ends.  The nine words sit
+// in LDS (rt: the lane's first, row r at rt[r * lstride]) and a row change
+// reads the next one; the row starts shift down a register queue.  On the
+// 1.64M scene (delta us), with the words in a register queue too: 134 with
+// 4 bits per batch, 141 / 137 with 3 / 6; walked row by row, the wave
+// waiting at each row's end for its lane with the most bits, 152 (3 / 6 / 8
+// bits: 156 / 159 / 165), and with only the next row's word loaded ahead
+// 155; a z-plane of three rows taken as one run, a batch's bits picked
+// across the plane's rows by selects, 155-166; the 64-scene batch 565
+// against 592 row by row.  The words in LDS instead of the queue (a row
+// change one LDS read instead of shifting eighteen registers): 131 -> 120 us
+// (VALU instructions per wave 3907 -> 3267, share of wave cycles waiting on
+// a dependency 41% -> 26%), the batch 558 -> 536.
 template <int B, class LD, class F>
 __device__ __forceinline__ void visit_masked_lds(const Grid& g, const int32_t* __restrict__ off, int bx, int by,
                                                  int bz, int t, int n, const uint64_t* __restrict__ rows,
@@ -1773,7 +1726,7 @@ __device__ __forceinline__ bool visit_quad_record(const Grid& g, const int32_t* 
 // quad folds the terms in slot order as visit_quad does; B rounds of loads in
 // flight, all from one row.  Each quad runs on to its next row without
 // waiting for the rest of its wave (the nine words and first slots shift down
-// a queue, as in visit_masked).  A lane past the row's last bit loads the
+// a register queue).  A lane past the row's last bit loads the
 // row's first slot and its term is the zero term (tm sees valid = false).
 // Dam break (25,600 particles), delta us: 17.5 / 17.7 / 20.8 with 2 / 1 / 4
 // rounds per batch; row by row with the wave in step (4 rounds) 20.3.
@@ -2227,12 +2180,9 @@ __device__ __forceinline__ void delta_body(int gt_in, int n, Grid g_, Consts k_,
             return false;
           };
           auto ld32 = [&](int q) { return slot_load(pred_in, q); };
-          if (pm.whole[t] == 0) {
-            if (rt)
-              visit_masked_lds<4>(g, off, bx, by, bz, t, n, pm.rows, rt, (int)blockDim.x, ld32, body_m);
-            else
-              visit_masked<4>(g, off, bx, by, bz, t, n, pm.rows, ld32, body_m);
-          }
+          static_assert((V & kLdsWords) != 0, "the one-lane walk over the pair masks keeps its words in LDS");
+          if (pm.whole[t] == 0)
+            visit_masked_lds<4>(g, off, bx, by, bz, t, n, pm.rows, rt, (int)blockDim.x, ld32, body_m);
           else
             visit_auto<MODE | VISIT_STREAM | VISIT_ROW_BATCH | ((V & 8) ? VISIT_PACKED : 0)>(g, off, k.max_scan, bx, by, bz, t, n, L, ld, body);
         } else if constexpr ((V & kLdsWalk) != 0)
