@@ -451,9 +451,11 @@ __global__ __launch_bounds__(256) void k_finalize_predict(int n, FxParams p, Col
                                                           float4* __restrict__ pos_r, float4* __restrict__ pred_r,
                                                           int32_t* __restrict__ key, int32_t* __restrict__ cnt,
                                                           int32_t* __restrict__ rank,
-                                                          float4* __restrict__ nrm_canon) {
+                                                          float4* __restrict__ nrm_canon,
+                                                          int32_t* __restrict__ big_count) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = t < n;
+  if (t == 0) *big_count = 0;  // the next build's big-cell queue (k_scatter_pairs)
   int k = 0;
   if (active) {
     const float4 pi = pred_s[t], vi = vel_s[t], dv = dv_s[t];
@@ -482,15 +484,17 @@ __global__ __launch_bounds__(256) void k_finalize_predict(int n, FxParams p, Col
 // queue's count (flags[0]) for the canonicalisation that follows and marks
 // the neighbour lists invalid (flags[2] = kInfBits): three fills fewer.
 constexpr int32_t kInfBits = 0x7f800000;  // +inf: neighbour lists invalid
+// cell runs longer than this are sorted by a workgroup (k_sort_big_cells);
+// shorter ones in registers or, in a resident build, ranked by the gather
+constexpr int kSmallCell = 32;
 
 __global__ void k_scatter_pairs(int n, const int32_t* __restrict__ key, const int32_t* __restrict__ rank,
                                 const int32_t* __restrict__ off, const int32_t* __restrict__ pid,
                                 uint64_t* __restrict__ pairs, int32_t* __restrict__ cnt, int n_cnt,
-                                int32_t* __restrict__ flags) {
+                                int32_t* __restrict__ flags, int32_t* __restrict__ big_list) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n_cnt) cnt[t] = 0;
   if (t == 0) {
-    flags[0] = 0;
     flags[2] = kInfBits;
     // the workspace header: this grid, and no normals by particle (the
     // resident substeps keep theirs by slot)
@@ -501,7 +505,11 @@ __global__ void k_scatter_pairs(int n, const int32_t* __restrict__ key, const in
     flags[kHdrError] = 0;
   }
   if (t >= n) return;
-  pairs[off[key[t]] + rank[t]] = ((uint64_t)(uint32_t)pid[t] << 32) | (uint32_t)t;
+  const int k = key[t], r = rank[t];
+  pairs[off[k] + r] = ((uint64_t)(uint32_t)pid[t] << 32) | (uint32_t)t;
+  // a cell's (kSmallCell + 1)-th member queues it for the block sort (the
+  // count was zeroed by the k_finalize_predict before this build)
+  if (r == kSmallCell) big_list[atomicAdd(&flags[0], 1)] = k;
 }
 
 // The closing mark of a grid build: the neighbour lists are invalid
@@ -549,18 +557,31 @@ __device__ __forceinline__ void soa_store(const SoaPos& a, int t, float x, float
 // dl_s (zeroed: read before any lambda pass only with no iterations) and the
 // sorted keys skey_int (read by the lists, the LDS-tiled walk and, after the
 // call, the splat) are skipped when null.
-__global__ void k_gather_pairs(int n, const uint64_t* __restrict__ pairs, const int32_t* __restrict__ key_r,
+__global__ void k_gather_pairs(int n, const uint64_t* __restrict__ pairs, const int32_t* __restrict__ off,
+                               const int32_t* __restrict__ key_r,
                                const float4* __restrict__ pred_r, const float4* __restrict__ pos_r,
                                float4* __restrict__ pred_s, float4* __restrict__ pos_s,
                                float2* __restrict__ dl_s, int32_t* __restrict__ skey_out,
                                int32_t* __restrict__ skey_int, int32_t* __restrict__ sidx,
                                int32_t* __restrict__ sidx_out, int32_t* __restrict__ key_canon,
                                SoaPos soa = SoaPos{nullptr, nullptr, nullptr}) {
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= n) return;
-  const uint64_t e = pairs[u];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint64_t e = pairs[v];
   const int pid = min(max((int)(e >> 32), 0), n - 1);
   const int src = min(max((int)(uint32_t)e, 0), n - 1);
+  // the pair's canonical slot: a run of up to kSmallCell pairs sits in
+  // scatter order and each pair ranks itself by particle index; a longer run
+  // was sorted in place by k_sort_big_cells
+  const int kc = key_r[src];
+  const int c0 = off[kc], m = off[kc + 1] - c0;
+  int u = v;
+  if (m <= kSmallCell) {
+    int r = 0;
+#pragma unroll 4
+    for (int j = 0; j < m; ++j) r += (uint32_t)(pairs[c0 + j] >> 32) < (uint32_t)pid ? 1 : 0;
+    u = c0 + r;
+  }
   float4 q = pred_r[src];
   q.w = 0.0f;
   pred_s[u] = q;
@@ -569,12 +590,9 @@ __global__ void k_gather_pairs(int n, const uint64_t* __restrict__ pairs, const 
   if (dl_s) dl_s[u] = make_float2(0.0f, 0.0f);
   sidx[u] = pid;
   if (sidx_out) sidx_out[u] = pid;
-  if (skey_int || skey_out || key_canon) {
-    const int k = key_r[src];
-    if (skey_int) skey_int[u] = k;
-    if (skey_out) skey_out[u] = k;
-    if (key_canon) key_canon[pid] = k;
-  }
+  if (skey_int) skey_int[u] = kc;
+  if (skey_out) skey_out[u] = kc;
+  if (key_canon) key_canon[pid] = kc;
 }
 
 // Canonicalise each cell's run to ascending particle index.  Runs of up to
@@ -584,7 +602,6 @@ __global__ void k_gather_pairs(int n, const uint64_t* __restrict__ pairs, const 
 // per-thread insertion sort (dependent loads, quadratic in the run) a few
 // dense cells set the time - 73 us at 1.64M particles for 8-byte pairs, 43 us
 // at 25,600 particles.
-constexpr int kSmallCell = 32;
 constexpr int kBigCellBlocks = 256;  // workgroups striding over the queued runs
 
 template <int N, class T>
@@ -2859,18 +2876,6 @@ int sort_queued_runs(const int32_t* off, int32_t* data, const int32_t* queue, co
   return FX_OK;
 }
 
-// The same for the (particle << 32 | previous slot) pairs of a resident
-// grid build: ordering the pairs orders the particles.  big_count must be
-// zero on entry (k_scatter_pairs clears it).
-static int canon_pairs(const int32_t* off, uint64_t* pairs, int n_cells, int32_t* big_list, int32_t* big_count,
-                       hipStream_t st) {
-  if (n_cells <= 0) return FX_OK;
-  LAUNCH(k_canon_cells<uint64_t>, n_cells, st, n_cells, off, pairs, big_list, big_count);
-  FX_LAUNCH(k_sort_big_cells<uint64_t>, dim3(kBigCellBlocks), dim3(256), st, off, pairs, (const int32_t*)big_list,
-            (const int32_t*)big_count);
-  return FX_OK;
-}
-
 // Counting sort of ws.key (counts already in ws.cnt, ranks in ws.rank) into
 // ws.sidx: scan, scatter (which zeroes the counts again), canonicalise.  Writes reference-format ranges if
 // requested.
@@ -2917,7 +2922,10 @@ static int build_grid(const FxParams& p, FxState& s, const Ws& w, bool do_predic
 // cnt).  The counting sort carries (particle, previous slot) pairs, so each
 // cell's run is still put in ascending particle order - the same order as
 // build_grid - and the gather reads the previous slots nearly in sequence
-// instead of gathering the canonical arrays.  The reference-format outputs
+// instead of gathering the canonical arrays.  The gather ranks each pair
+// within its cell's run itself (runs of up to kSmallCell pairs; longer ones
+// are sorted in place first): 25 us of canonicalisation and a 39 us gather
+// became a 49 us gather at 1.64M.  The reference-format outputs
 // (keys, sorted keys and indices, cell ranges) go to the caller on the
 // call's last substep only.
 static int build_grid_resident(const FxParams& p, FxState& s, const Ws& w, int target, bool last_in_call,
@@ -2927,16 +2935,18 @@ static int build_grid_resident(const FxParams& p, FxState& s, const Ws& w, int t
   FX_TRY(exclusive_scan_i32(w.cnt, w.off, nc, w.partial, st));
   if (last_in_call && s.cell_start && s.cell_end) LAUNCH(k_cell_ranges, nc, st, nc, w.off, s.cell_start, s.cell_end);
   // also zeroes the counts (nc + 1) and flags[0], and invalidates the lists
+  // big-cell queue in the neighbour lists' cell array (the lists are off
+  // in a resident build); its count was zeroed by k_finalize_predict
   LAUNCH(k_scatter_pairs, std::max(n, nc + 1), st, n, (const int32_t*)w.key, (const int32_t*)w.rank,
-         (const int32_t*)w.off, (const int32_t*)w.sidx, w.pair64, w.cnt, nc + 1, w.flags);
-  // big-cell queue in the rank array (dead after the scatter)
-  FX_TRY(canon_pairs(w.off, w.pair64, nc, w.rank, w.flags, st));
+         (const int32_t*)w.off, (const int32_t*)w.sidx, w.pair64, w.cnt, nc + 1, w.flags, w.lcell);
+  FX_LAUNCH(k_sort_big_cells<uint64_t>, dim3(kBigCellBlocks), dim3(256), st, (const int32_t*)w.off, w.pair64,
+            (const int32_t*)w.lcell, (const int32_t*)w.flags);
   // the sorted keys: for the lists and the LDS-tiled walk, and after the
   // call's last substep for the splat (its per-cell displacement bounds bin
   // each slot by its key)
   const bool lists = g_tuning.skin > 0.0f;
   const bool skey = lists || g_tuning.walk == 3 || last_in_call;
-  LAUNCH(k_gather_pairs, n, st, n, (const uint64_t*)w.pair64, (const int32_t*)w.key, pred_r,
+  LAUNCH(k_gather_pairs, n, st, n, (const uint64_t*)w.pair64, (const int32_t*)w.off, (const int32_t*)w.key, pred_r,
          (const float4*)w.pos_r, w.pred_s[target], w.pos_s, p.iterations > 0 ? (float2*)nullptr : w.dl_s,
          last_in_call ? s.sorted_key : nullptr, skey ? w.skey : (int32_t*)nullptr, w.sidx,
          last_in_call ? s.sorted_idx : nullptr, last_in_call ? s.key : nullptr, xyz_arrays(w, n, lists));
@@ -3079,7 +3089,7 @@ static int substep(const FxParams& p, FxState& s, const Ws& w, hipStream_t st, b
       LAUNCH(k_finalize_predict, n, st, n, p, col, g, (const float4*)w.pred_s[0], (const float4*)w.vel_s,
              (const float4*)w.pred_s[1], cohesion ? (const float4*)w.normal_next : (const float4*)nullptr,
              (const int32_t*)w.sidx, w.pos_r, w.vel_r, w.key, w.cnt, w.rank,
-             (cohesion && next_is_last) ? (float4*)s.normal : (float4*)nullptr);
+             (cohesion && next_is_last) ? (float4*)s.normal : (float4*)nullptr, w.flags);
     } else {
       // with fused normals, the header records where they went
       const NormalMark mk = fuse_normals ? NormalMark{w.flags, last_in_call ? 1 : 2, normal_key(p, s.normal)}
